@@ -1,0 +1,174 @@
+/*
+ * C ABI of the MI355X GPU-driven frame (liblh.so).
+ *
+ * The reference records its frame with Vulkan in render()
+ * (src/rendering.h:4-16, src/rendering.cpp:28-548); every pass there is a
+ * compute dispatch or an indirect draw that reads the shared structs through
+ * buffer-device addresses and push constants
+ * {u64 uniforms_address; u32 cascade_index} (src/shaders/common/bindings.glsl:23-26).
+ * Here every pass is an extern "C" function taking
+ *   - lh_scene: device addresses of the static scene (instances, mesh infos,
+ *     meshlet tables, textures) - the reference's InstanceBuffer /
+ *     MeshInfoBuffer / bindless texture table;
+ *   - lh_view: device addresses of one view's working set (uniforms, misc
+ *     storage, prefix-sum sectors, draw lists, visibility buffer, shadow maps,
+ *     radiance) - the reference's per-frame resources;
+ *   - a HIP stream (passed as void*); passes are stream ordered, so the
+ *     reference's pipeline barriers become plain stream order.
+ * Nothing here allocates device memory: all buffers are caller-owned, so a
+ * sequence of calls can be captured into a hipGraph.  All structs hold plain
+ * device pointers and sizes; no C++ or torch types cross the boundary.
+ *
+ * Pass numbering: pass 0 = main view, pass 1 + c = shadow cascade c (c < 4).
+ */
+#ifndef LH_LH_H
+#define LH_LH_H
+
+#include "shared_cpu_gpu.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef int32_t lh_status;
+#define LH_OK 0
+#define LH_ERR_INVALID 1     /* bad argument / size out of range */
+#define LH_ERR_CAPACITY 2    /* a capacity-bounded buffer overflowed (see counters) */
+#define LH_ERR_HIP 3         /* a HIP runtime call failed */
+#define LH_ERR_UNSUPPORTED 4 /* feature outside this build (e.g. alpha clip) */
+
+typedef void* lh_stream; /* hipStream_t */
+
+#define LH_NUM_PASSES 5
+#define LH_NUM_CASCADES 4
+#define LH_MAX_VIEW_SIDE 8192u
+
+/* Device counters of one view (u32 each). */
+#define LH_CNT_DRAWS 0     /* + pass: surviving meshlets of the pass (may exceed draw_capacity) */
+#define LH_CNT_BIG 8       /* + pass: triangles queued for the large-triangle raster */
+#define LH_CNT_CLIP 16     /* + pass: triangles queued for clipping */
+#define LH_CNT_OVERFLOW 24 /* bit mask of LH_OVERFLOW_* */
+#define LH_CNT_FLAGS 25    /* bit mask of LH_FLAG_* */
+#define LH_NUM_COUNTERS 64
+#define LH_OVERFLOW_INSTANCES 0x1u
+#define LH_OVERFLOW_DRAWS 0x2u
+#define LH_OVERFLOW_BIG 0x4u
+#define LH_OVERFLOW_CLIP 0x8u
+#define LH_FLAG_EMPTY_VIEW 0x1u /* no geometry: depth bounds stayed at their reset values */
+
+/* Bytes per entry of the two raster queues inside lh_view.scratch:
+ * [clip queue: clip_capacity x 4 B][pad to 256][big queue: big_capacity x 48 B]. */
+#define LH_CLIP_ENTRY_BYTES 4u
+#define LH_BIG_ENTRY_BYTES 48u
+
+typedef struct lh_scene {
+    const Instance* instances;              /* [num_instances] */
+    const MeshInfo* mesh_infos;             /* [num_mesh_infos]; Instance.mesh_info_address points here */
+    const LhMaterialFactors* factors;       /* [num_mesh_infos] */
+    const uint32_t* instance_meshlet_base;  /* [num_instances] exclusive scan of num_meshlets */
+    const MeshletReference* meshlet_table;  /* [total_meshlets] stable meshlet id -> (instance, meshlet) */
+    const LhTexture* textures;              /* [num_textures] */
+    uint32_t num_instances;
+    uint32_t num_mesh_infos;
+    uint32_t total_meshlets;
+    uint32_t num_textures;
+} lh_scene;
+
+typedef struct lh_view {
+    const Uniforms* uniforms;   /* device copy of this view's uniforms */
+    MiscStorage* misc;          /* cascade matrices + depth bounds */
+    uint64_t* prefix_sums;      /* LH_NUM_PASSES sectors of (1 + instance_capacity) u64 words:
+                                   word 0 = counter (count << 32 | sum), then PrefixSumValue[] */
+    MeshletReference* draws;    /* LH_NUM_PASSES lists of draw_capacity entries */
+    uint32_t* counters;         /* LH_NUM_COUNTERS */
+    uint64_t* visbuffer;        /* width * height keys */
+    uint32_t* shadow_maps;      /* LH_NUM_CASCADES * shadow_size^2 float bits */
+    float* radiance;            /* width * height * 4 (RGBA, FP32) */
+    uint32_t* ldr;              /* width * height BGRA8 after tonemap; may be NULL */
+    void* scratch;              /* raster queues, see LH_*_ENTRY_BYTES */
+    uint64_t scratch_bytes;
+    uint32_t instance_capacity;
+    uint32_t draw_capacity;
+    uint32_t width;
+    uint32_t height;
+    uint32_t shadow_size;       /* cascade side (reference: 1024, src/main.cpp:385-409) */
+    uint32_t big_capacity;
+    uint32_t clip_capacity;
+    uint32_t flags;             /* reserved, 0 */
+} lh_view;
+
+/* Bytes of scratch needed for the given queue capacities. */
+uint64_t lh_scratch_bytes(uint32_t big_capacity, uint32_t clip_capacity);
+
+/* reset_buffers_a (src/shaders/compute/reset_buffers.glsl:9-29) + the
+ * attachment clears of src/rendering.cpp:206-211,384: counters and prefix-sum
+ * sectors to 0, min_depth = UINT_MAX, max_depth = 0, visibility buffer to
+ * LH_VISBUFFER_CLEAR, shadow maps to LH_SHADOW_CLEAR. */
+lh_status lh_view_reset(const lh_scene* scene, const lh_view* view, lh_stream stream);
+
+/* cull_instances (src/shaders/cull_instances.comp:7-27): sphere frustum test,
+ * 64-bit-atomic inclusive prefix-sum append of num_meshlets into sector 0
+ * (src/shaders/common/prefix_sum.glsl:17-30). */
+lh_status lh_cull_instances(const lh_scene* scene, const lh_view* view, lh_stream stream);
+
+/* write_draw_calls (pass 0, src/shaders/write_draw_calls.comp:7-57) and
+ * write_draw_calls_shadows (pass 1 + c, :61-117): per meshlet of the visible
+ * instances, sphere + normal-cone culling and compaction into draws[pass]. */
+lh_status lh_cull_meshlets(const lh_scene* scene, const lh_view* view, uint32_t pass, lh_stream stream);
+
+/* Visibility-buffer raster of draws[0] (replaces visbuffer_opaque_vertex/pixel,
+ * src/shaders/rasterization.glsl:38-58, drawn at src/rendering.cpp:235-242):
+ * 64-bit atomicMin of depth_key << 32 | meshlet << 7 | triangle. */
+lh_status lh_raster_view(const lh_scene* scene, const lh_view* view, lh_stream stream);
+
+/* read_depth (src/shaders/compute/read_depth.glsl:22-83): min over non-zero
+ * and max of the depth bits into misc->min_depth / max_depth. */
+lh_status lh_depth_bounds(const lh_scene* scene, const lh_view* view, lh_stream stream);
+
+/* generate_shadow_matrices (src/shaders/compute/generate_shadow_matrices.glsl:15-110). */
+lh_status lh_fit_cascades(const lh_scene* scene, const lh_view* view, lh_stream stream);
+
+/* cull_instances_shadows (src/shaders/cull_instances.comp:31-56): all four
+ * cascades, appending into sectors 1..4. */
+lh_status lh_cull_instances_shadow(const lh_scene* scene, const lh_view* view, lh_stream stream);
+
+/* Depth-only shadow raster of draws[1 + cascade] (shadowmap_opaque_vertex,
+ * src/shaders/rasterization.glsl:101-111; front faces culled, compare LESS). */
+lh_status lh_raster_shadow(const lh_scene* scene, const lh_view* view, uint32_t cascade, lh_stream stream);
+
+/* render_geometry (src/shaders/render_geometry.comp:70-266): visibility-buffer
+ * resolve into FP32 radiance; when view->ldr is non-NULL the display
+ * transform (src/shaders/display_transform.comp:37-81) is fused in. */
+lh_status lh_resolve(const lh_scene* scene, const lh_view* view, lh_stream stream);
+
+/* display_transform as a separate pass: radiance -> BGRA8 ldr. */
+lh_status lh_tonemap(const lh_view* view, lh_stream stream);
+
+/* The whole frame in the reference's pass order (src/rendering.cpp:79-502). */
+lh_status lh_frame(const lh_scene* scene, const lh_view* view, lh_stream stream);
+
+/* Copies the view's counters to host memory (synchronizes the stream) and
+ * returns LH_ERR_CAPACITY if any overflow bit is set. */
+lh_status lh_view_status(const lh_view* view, uint32_t* host_counters, lh_stream stream);
+
+/* Launch tunables of the culling and raster kernels (no counterpart in the
+ * reference, whose raster is fixed-function).  Results are independent of
+ * them; the autotuner picks them per resolution class.  Process-global. */
+typedef struct lh_tuning {
+    uint32_t small_extent_px; /* triangles whose vertex bbox fits in this many pixels (1..32) are
+                                 rasterized by one lane; larger ones cooperatively */
+    uint32_t meshlet_blocks;  /* persistent grid of the per-meshlet raster (one wave per block) */
+    uint32_t big_blocks;      /* persistent grid of the large-triangle raster */
+    uint32_t clip_blocks;     /* persistent grid of the clipping kernel */
+    uint32_t cull_blocks;     /* persistent grid of the meshlet cull */
+} lh_tuning;
+lh_status lh_set_tuning(const lh_tuning* tuning);
+lh_status lh_get_tuning(lh_tuning* tuning);
+
+/* Library version string. */
+const char* lh_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* LH_LH_H */

@@ -1,0 +1,252 @@
+// Culling passes of the GPU-driven frame (gfx950).
+//
+//   k_view_reset          reset_buffers_a    src/shaders/compute/reset_buffers.glsl:9-29
+//   k_fill_u64/u32        attachment clears  src/rendering.cpp:206-211, :384
+//   k_cull_instances      cull_instances     src/shaders/cull_instances.comp:7-27
+//                         cull_instances_shadows               :31-56
+//   k_cull_meshlets       write_draw_calls(_shadows)  src/shaders/write_draw_calls.comp:7-117
+//
+// The prefix-sum append keeps the reference's single 64-bit atomic per append
+// (count in the high word, running sum in the low word,
+// src/shaders/common/prefix_sum.glsl:17-30) but aggregates a whole wave into
+// one atomic: lanes take consecutive slots in lane order and their inclusive
+// sums from a wave scan, so the sector has exactly the invariants of
+// one-by-one appends (sums strictly increasing with the slot).
+#include "lh_device.h"
+#include "kernels.h"
+
+using namespace lhd;
+
+namespace {
+
+// cull_bounding_sphere (src/shaders/common/culling.glsl:1-46).
+LH_DEV void world_sphere(const Instance& in, const LhVec4& s, f3& c, float& r) {
+    c = mat4_point(in.transform, f3{s.x, s.y, s.z});
+    float sx = length3(f3{in.transform.c[0].x, in.transform.c[0].y, in.transform.c[0].z});
+    float sy = length3(f3{in.transform.c[1].x, in.transform.c[1].y, in.transform.c[1].z});
+    float sz = length3(f3{in.transform.c[2].x, in.transform.c[2].y, in.transform.c[2].z});
+    r = s.w * fmaxf(fmaxf(sx, sy), sz);
+}
+
+LH_DEV bool cull_sphere(const Instance& in, const LhVec4& s, const Uniforms& u) {
+    f3 c;
+    float r;
+    world_sphere(in, s, c, r);
+    f3 v = mat4_point(u.initial_view, c);
+    v.z = -v.z;
+    bool visible = v.z + r > near_plane(u);
+    const LhMat4& p = u.perspective;
+    f3 row3{p.c[0].w, p.c[1].w, p.c[2].w}, row0{p.c[0].x, p.c[1].x, p.c[2].x}, row1{p.c[0].y, p.c[1].y, p.c[2].y};
+    f3 fx = normalize3(add(row3, row0));
+    f3 fy = normalize3(add(row3, row1));
+    visible = visible && (v.z * fx.z + fabsf(v.x) * fx.x < r);
+    visible = visible && (v.z * fy.z - fabsf(v.y) * fy.y < r);
+    return !visible;
+}
+
+LH_DEV f2 shadow_view_xy(const MiscStorage& m, uint32_t c, f3 w) {
+    f3 v = mat4_point(m.shadow_view_matrices[c], w);
+    return {v.x, v.y};
+}
+
+// cull_bounding_sphere_shadows (culling.glsl:82-135) with
+// fits_partially_inside / fits_entirely_inside(_cascade) (:48-80).
+LH_DEV bool cull_sphere_shadow(const Instance& in, const LhVec4& s, const MiscStorage& m, uint32_t c) {
+    f3 w;
+    float r;
+    world_sphere(in, s, w, r);
+    f2 p = shadow_view_xy(m, c, w);
+    float R = m.shadow_sphere_radii[c];
+    if (!((fabsf(p.x) - r < R) && (fabsf(p.y) - r < R))) return true;
+    for (uint32_t k = 0; k < c; k++) {
+        f2 q = shadow_view_xy(m, k, w);
+        float Rk = m.shadow_sphere_radii[k];
+        if ((fabsf(q.x) + r < Rk) && (fabsf(q.y) + r < Rk)) return true;
+    }
+    return false;
+}
+
+// cull_cone_perspective / cull_cone_orthographic (culling.glsl:137-151).
+LH_DEV bool cull_cone(const Instance& in, const Meshlet& ml, const Uniforms& u, bool ortho) {
+    f3 axis = normalize3(mat3_mul_vec3(in.normal_transform, mk3(ml.cone_axis)));
+    if (ortho) return dot3(mk3(u.sun_dir), axis) >= ml.cone_cutoff;
+    f3 apex = mat4_point(in.transform, mk3(ml.cone_apex));
+    return dot3(normalize3(sub(apex, mk3(u.camera_pos))), normalize3(axis)) >= ml.cone_cutoff;
+}
+
+// Wave-aggregated prefix_sum_inclusive_append.  Must be reached by every lane
+// of the wave (lanes without an element pass visible = false).
+LH_DEV void wave_prefix_append(const lh_view& v, uint32_t pass, bool visible, uint32_t index, uint32_t value) {
+    uint64_t m = ballot(visible);
+    if (m == 0) return;
+    uint32_t incl = wave_inclusive_scan(visible ? value : 0u);
+    uint32_t total = shfl(incl, 63);
+    uint32_t leader = first_lane(m);
+    uint64_t* sec = sector(v, pass);
+    unsigned long long old = 0;
+    if (lane_id() == leader)
+        old = atomicAdd(reinterpret_cast<unsigned long long*>(sec),
+                        (unsigned long long)total | ((unsigned long long)__popcll(m) << 32));
+    old = shfl(old, int(leader));
+    if (visible) {
+        uint32_t slot = uint32_t(old >> 32) + lanes_below(m);
+        if (slot < v.instance_capacity) {
+            PrefixSumValue* vals = reinterpret_cast<PrefixSumValue*>(sec + 1);
+            vals[slot] = PrefixSumValue{index, uint32_t(old) + incl};
+        } else {
+            atomicOr(&v.counters[LH_CNT_OVERFLOW], LH_OVERFLOW_INSTANCES);
+        }
+    }
+}
+
+// prefix_sum_binary_search (prefix_sum.glsl:33-46): upper_bound on sum.
+LH_DEV bool prefix_search(const uint64_t* sec, uint32_t n, uint32_t target, PrefixSumValue& out) {
+    const PrefixSumValue* vals = reinterpret_cast<const PrefixSumValue*>(sec + 1);
+    uint32_t first = 0, count = n;
+    while (count > 0) {
+        uint32_t step = count / 2, cur = first + step;
+        if (target >= vals[cur].sum) {
+            first = cur + 1;
+            count -= step + 1;
+        } else {
+            count = step;
+        }
+    }
+    if (first >= n) return false;
+    out = vals[first];
+    return true;
+}
+
+}  // namespace
+
+__global__ void __launch_bounds__(64) k_view_reset(lh_view v) {
+    uint32_t t = threadIdx.x;
+    v.counters[t] = 0;  // LH_NUM_COUNTERS == 64
+    if (t < LH_NUM_PASSES) sector(v, t)[0] = 0;
+    if (t == 0) {
+        v.misc->min_depth = 0xFFFFFFFFu;
+        v.misc->max_depth = 0u;
+    }
+}
+
+__global__ void __launch_bounds__(256) k_fill_u64(uint64_t* p, uint64_t n, uint64_t value) {
+    // Two keys per store (16 B); n is even for every caller except the tail.
+    uint64_t i = (uint64_t(blockIdx.x) * blockDim.x + threadIdx.x) * 2;
+    uint64_t stride = uint64_t(gridDim.x) * blockDim.x * 2;
+    for (; i + 1 < n; i += stride) {
+        ulonglong2 val = make_ulonglong2(value, value);
+        *reinterpret_cast<ulonglong2*>(p + i) = val;
+    }
+    if (i < n) p[i] = value;
+}
+
+__global__ void __launch_bounds__(256) k_fill_u32(uint32_t* p, uint64_t n, uint32_t value) {
+    uint64_t i = (uint64_t(blockIdx.x) * blockDim.x + threadIdx.x) * 4;
+    uint64_t stride = uint64_t(gridDim.x) * blockDim.x * 4;
+    for (; i + 3 < n; i += stride) *reinterpret_cast<uint4*>(p + i) = make_uint4(value, value, value, value);
+    for (; i < n; i++) p[i] = value;
+}
+
+// Main view (shadow = false, sector 0) or cascade blockIdx.y (sector 1 + c).
+template <bool SHADOW>
+__global__ void __launch_bounds__(256) k_cull_instances(lh_scene s, lh_view v) {
+    const Uniforms& u = *v.uniforms;
+    uint32_t n = min(u.num_instances, s.num_instances);
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t cascade = SHADOW ? blockIdx.y : 0u;
+    bool visible = false;
+    uint32_t value = 0;
+    if (i < n) {
+        const Instance& in = s.instances[i];
+        const MeshInfo& mi = mesh_info_at(in.mesh_info_address);
+        value = mi.num_meshlets;
+        visible = SHADOW ? !cull_sphere_shadow(in, mi.bounding_sphere, *v.misc, cascade)
+                         : !cull_sphere(in, mi.bounding_sphere, u);
+    }
+    wave_prefix_append(v, SHADOW ? 1 + cascade : 0, visible, i, value);
+}
+
+// One pass per blockIdx.y (pass = pass0 + blockIdx.y); grid-stride over the
+// meshlets of the pass's visible instances.
+__global__ void __launch_bounds__(256) k_cull_meshlets(lh_scene s, lh_view v, uint32_t pass0) {
+    const uint32_t pass = pass0 + blockIdx.y;
+    const Uniforms& u = *v.uniforms;
+    const uint64_t* sec = sector(v, pass);
+    const uint64_t head = sec[0];
+    const uint32_t total = uint32_t(head);
+    const uint32_t n = min(uint32_t(head >> 32), v.instance_capacity);
+    MeshletReference* out = v.draws + size_t(pass) * v.draw_capacity;
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t base = blockIdx.x * blockDim.x; base < total; base += stride) {
+        uint32_t id = base + threadIdx.x;
+        bool survive = false;
+        PrefixSumValue e{0, 0};
+        uint32_t local = 0;
+        if (id < total && prefix_search(sec, n, id, e)) {
+            const Instance& in = s.instances[e.index];
+            const MeshInfo& mi = mesh_info_at(in.mesh_info_address);
+            local = id - (e.sum - mi.num_meshlets);
+            const Meshlet& ml = ptr<Meshlet>(mi.meshlets)[local];
+            bool alpha = (mi.flags & MESH_INFO_FLAGS_ALPHA_CLIP) != 0;
+            if (pass == 0) {
+                survive = !cull_sphere(in, ml.bounding_sphere, u) && (alpha || !cull_cone(in, ml, u, false));
+            } else {
+                survive = !cull_sphere_shadow(in, ml.bounding_sphere, *v.misc, pass - 1) &&
+                          (alpha || !cull_cone(in, ml, u, true));
+            }
+        }
+        uint32_t slot = wave_append(&v.counters[LH_CNT_DRAWS + pass], survive);
+        if (survive) {
+            if (slot < v.draw_capacity) {
+                MeshletReference r;
+                r.instance_index = e.index;
+                r.meshlet_index = uint16_t(local);
+                out[slot] = r;
+            } else {
+                atomicOr(&v.counters[LH_CNT_OVERFLOW], LH_OVERFLOW_DRAWS);
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------- launchers
+
+namespace lhk {
+
+hipError_t view_reset(const lh_view& v, hipStream_t st) {
+    hipLaunchKernelGGL(k_view_reset, dim3(1), dim3(64), 0, st, v);
+    return hipGetLastError();
+}
+
+hipError_t fill_u64(uint64_t* p, uint64_t n, uint64_t value, hipStream_t st) {
+    if (n == 0) return hipSuccess;
+    uint64_t blocks = std::min<uint64_t>((n / 2 + 255) / 256 + 1, 8192);
+    hipLaunchKernelGGL(k_fill_u64, dim3(uint32_t(blocks)), dim3(256), 0, st, p, n, value);
+    return hipGetLastError();
+}
+
+hipError_t fill_u32(uint32_t* p, uint64_t n, uint32_t value, hipStream_t st) {
+    if (n == 0) return hipSuccess;
+    uint64_t blocks = std::min<uint64_t>((n / 4 + 255) / 256 + 1, 8192);
+    hipLaunchKernelGGL(k_fill_u32, dim3(uint32_t(blocks)), dim3(256), 0, st, p, n, value);
+    return hipGetLastError();
+}
+
+hipError_t cull_instances(const lh_scene& s, const lh_view& v, bool shadow, hipStream_t st) {
+    uint32_t n = s.num_instances;
+    if (n == 0) return hipSuccess;
+    dim3 grid((n + 255) / 256, shadow ? LH_NUM_CASCADES : 1);
+    if (shadow)
+        hipLaunchKernelGGL(k_cull_instances<true>, grid, dim3(256), 0, st, s, v);
+    else
+        hipLaunchKernelGGL(k_cull_instances<false>, grid, dim3(256), 0, st, s, v);
+    return hipGetLastError();
+}
+
+hipError_t cull_meshlets(const lh_scene& s, const lh_view& v, uint32_t pass0, uint32_t npasses, uint32_t blocks,
+                         hipStream_t st) {
+    hipLaunchKernelGGL(k_cull_meshlets, dim3(blocks, npasses), dim3(256), 0, st, s, v, pass0);
+    return hipGetLastError();
+}
+
+}  // namespace lhk

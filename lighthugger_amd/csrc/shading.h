@@ -1,0 +1,323 @@
+// Per-pixel shading of the visibility-buffer resolve (device code), shared by
+// the forward resolve and its adjoint.
+//
+// Follows src/shaders/render_geometry.comp:70-266 with CalcFullBary and the
+// interpolation helpers of src/shaders/common/vbuffer.glsl:11-119 and the
+// BRDF of src/shaders/common/pbr.glsl:1-96.
+#pragma once
+
+#include "lh_device.h"
+
+namespace shading {
+
+using namespace lhd;
+
+constexpr float kPi = 3.14159265358979323846f;
+// Constant sky radiance: base colour times 2^3 exposure
+// (render_geometry.comp:93-97; the skybox cubemap is absent).
+constexpr float kSkyR = 0.35f * 8.0f, kSkyG = 0.50f * 8.0f, kSkyB = 0.80f * 8.0f;
+
+LH_DEV uint32_t load_index(const MeshInfo& mi, uint32_t i) {
+    if (mi.flags & MESH_INFO_FLAGS_32_BIT_INDICES) return ptr<uint32_t>(mi.indices)[i];
+    return ptr<uint16_t>(mi.indices)[i];
+}
+LH_DEV f3 load_position(const MeshInfo& mi, uint32_t v) {
+    if (mi.flags & MESH_INFO_FLAGS_FP32_VERTICES) {
+        const float* q = ptr<float>(mi.positions) + 3 * size_t(v);
+        return {q[0], q[1], q[2]};
+    }
+    const uint16_t* q = ptr<uint16_t>(mi.positions) + 3 * size_t(v);
+    return {float(q[0]), float(q[1]), float(q[2])};
+}
+LH_DEV f3 load_normal(const MeshInfo& mi, uint32_t v) {
+    if (mi.flags & MESH_INFO_FLAGS_FP32_VERTICES) {
+        const float* q = ptr<float>(mi.normals) + 3 * size_t(v);
+        return {q[0], q[1], q[2]};
+    }
+    const int8_t* q = ptr<int8_t>(mi.normals) + 3 * size_t(v);
+    return {float(q[0]), float(q[1]), float(q[2])};
+}
+LH_DEV f2 load_uv(const MeshInfo& mi, uint32_t v) {
+    if (mi.flags & MESH_INFO_FLAGS_FP32_VERTICES) {
+        const float* q = ptr<float>(mi.uvs) + 2 * size_t(v);
+        return {q[0], q[1]};
+    }
+    const uint16_t* q = ptr<uint16_t>(mi.uvs) + 2 * size_t(v);
+    return {float(q[0]), float(q[1])};
+}
+
+struct BaryDeriv {
+    f3 lambda, ddx, ddy;
+};
+
+// CalcFullBary (src/shaders/common/vbuffer.glsl:11-63).
+LH_DEV BaryDeriv calc_full_bary(f4 p0, f4 p1, f4 p2, f2 ndc, f2 win) {
+    BaryDeriv r;
+    const f3 invW{1.0f / p0.w, 1.0f / p1.w, 1.0f / p2.w};
+    const f2 n0{p0.x * invW.x, p0.y * invW.x}, n1{p1.x * invW.y, p1.y * invW.y}, n2{p2.x * invW.z, p2.y * invW.z};
+    const f2 c0{n2.x - n1.x, n2.y - n1.y}, c1{n0.x - n1.x, n0.y - n1.y};
+    const float invDet = 1.0f / (c0.x * c1.y - c1.x * c0.y);
+    r.ddx = mul(mul(f3{n1.y - n2.y, n2.y - n0.y, n0.y - n1.y}, invDet), invW);
+    r.ddy = mul(mul(f3{n2.x - n1.x, n0.x - n2.x, n1.x - n0.x}, invDet), invW);
+    float ddxSum = (r.ddx.x + r.ddx.y) + r.ddx.z;
+    float ddySum = (r.ddy.x + r.ddy.y) + r.ddy.z;
+    const f2 dv{ndc.x - n0.x, ndc.y - n0.y};
+    const float interpInvW = (invW.x + dv.x * ddxSum) + dv.y * ddySum;
+    const float interpW = 1.0f / interpInvW;
+    r.lambda.x = interpW * ((invW.x + dv.x * r.ddx.x) + dv.y * r.ddy.x);
+    r.lambda.y = interpW * ((0.0f + dv.x * r.ddx.y) + dv.y * r.ddy.y);
+    r.lambda.z = interpW * ((0.0f + dv.x * r.ddx.z) + dv.y * r.ddy.z);
+    const float sx = 2.0f / win.x, sy = 2.0f / win.y;
+    r.ddx = mul(r.ddx, sx);
+    r.ddy = mul(r.ddy, sy);
+    ddxSum *= sx;
+    ddySum *= sy;
+    r.ddy = mul(r.ddy, -1.0f);
+    ddySum *= -1.0f;
+    const float wdx = 1.0f / (interpInvW + ddxSum), wdy = 1.0f / (interpInvW + ddySum);
+    r.ddx = f3{wdx * (r.lambda.x * interpInvW + r.ddx.x) - r.lambda.x,
+               wdx * (r.lambda.y * interpInvW + r.ddx.y) - r.lambda.y,
+               wdx * (r.lambda.z * interpInvW + r.ddx.z) - r.lambda.z};
+    r.ddy = f3{wdy * (r.lambda.x * interpInvW + r.ddy.x) - r.lambda.x,
+               wdy * (r.lambda.y * interpInvW + r.ddy.y) - r.lambda.y,
+               wdy * (r.lambda.z * interpInvW + r.ddy.z) - r.lambda.z};
+    return r;
+}
+
+LH_DEV f3 interp1(const BaryDeriv& d, float a, float b, float c) {
+    const f3 m{a, b, c};
+    return {dot3(m, d.lambda), dot3(m, d.ddx), dot3(m, d.ddy)};
+}
+struct Interp3 {
+    f3 v, dx, dy;
+};
+struct Interp2 {
+    f2 v, dx, dy;
+};
+LH_DEV Interp3 interp3(const BaryDeriv& d, f3 a, f3 b, f3 c) {
+    const f3 x = interp1(d, a.x, b.x, c.x), y = interp1(d, a.y, b.y, c.y), z = interp1(d, a.z, b.z, c.z);
+    return {{x.x, y.x, z.x}, {x.y, y.y, z.y}, {x.z, y.z, z.z}};
+}
+LH_DEV Interp2 interp2(const BaryDeriv& d, f2 a, f2 b, f2 c) {
+    const f3 x = interp1(d, a.x, b.x, c.x), y = interp1(d, a.y, b.y, c.y);
+    return {{x.x, y.x}, {x.y, y.y}, {x.z, y.z}};
+}
+
+LH_DEV f4 mix4(f4 a, f4 b, float t) { return {mixf(a.x, b.x, t), mixf(a.y, b.y, t), mixf(a.z, b.z, t), mixf(a.w, b.w, t)}; }
+
+LH_DEV f4 texel(const LhTexture& t, uint32_t level, int32_t i, int32_t j) {
+    const int32_t w = int32_t(max(1u, t.width >> level)), h = int32_t(max(1u, t.height >> level));
+    i %= w;
+    if (i < 0) i += w;
+    j %= h;
+    if (j < 0) j += h;
+    const uint32_t p = ptr<uint32_t>(t.texels)[size_t(t.level_offset[level]) + size_t(j) * w + i];
+    return {float(p & 255u) / 255.0f, float((p >> 8) & 255u) / 255.0f, float((p >> 16) & 255u) / 255.0f,
+            float(p >> 24) / 255.0f};
+}
+LH_DEV f4 bilinear(const LhTexture& t, uint32_t level, f2 uv) {
+    const float w = float(max(1u, t.width >> level)), h = float(max(1u, t.height >> level));
+    const float u = uv.x * w - 0.5f, v = uv.y * h - 0.5f;
+    const float fu = floorf(u), fv = floorf(v);
+    const float a = u - fu, b = v - fv;
+    const int32_t i = int32_t(fu), j = int32_t(fv);
+    return mix4(mix4(texel(t, level, i, j), texel(t, level, i + 1, j), a),
+                mix4(texel(t, level, i, j + 1), texel(t, level, i + 1, j + 1), a), b);
+}
+
+// sample_texture (render_geometry.comp:59-66) as isotropic trilinear
+// filtering with the LOD from the analytic uv derivatives.  An unused index
+// reads as opaque white.
+LH_DEV f4 sample_texture(const lh_scene& s, uint32_t index, const Interp2& uv) {
+    if (index == UNUSED_TEXTURE_INDEX || index >= s.num_textures) return {1, 1, 1, 1};
+    const LhTexture& t = s.textures[index];
+    const float W = float(t.width), H = float(t.height);
+    const float ax = uv.dx.x * W, ay = uv.dx.y * H, bx = uv.dy.x * W, by = uv.dy.y * H;
+    const float rho = fmaxf(sqrtf(ax * ax + ay * ay), sqrtf(bx * bx + by * by));
+    float lambda = rho > 0.0f ? log2f(rho) : 0.0f;
+    lambda = clampf(lambda, 0.0f, float(t.levels - 1));
+    const float d0 = floorf(lambda), f = lambda - d0;
+    const uint32_t l0 = uint32_t(d0), l1 = min(l0 + 1, t.levels - 1);
+    const f4 c0 = bilinear(t, l0, uv.v);
+    if (f == 0.0f) return c0;
+    return mix4(c0, bilinear(t, l1, uv.v), f);
+}
+
+// sampler2DArrayShadow tap: linear filter, clamp to edge, compare LESS.
+LH_DEV float shadow_tap(const lh_view& v, uint32_t layer, f2 uv, float ref) {
+    const uint32_t S = v.shadow_size;
+    const uint32_t* map = v.shadow_maps + size_t(layer) * S * S;
+    const float u = uv.x * float(S) - 0.5f, w = uv.y * float(S) - 0.5f;
+    const float fu = floorf(u), fw = floorf(w);
+    const float a = u - fu, b = w - fw;
+    const int32_t i = int32_t(fu), j = int32_t(fw);
+    auto cmp = [&](int32_t x, int32_t y) {
+        x = min(max(x, 0), int32_t(S) - 1);
+        y = min(max(y, 0), int32_t(S) - 1);
+        return ref < __uint_as_float(map[size_t(y) * S + x]) ? 1.0f : 0.0f;
+    };
+    return mixf(mixf(cmp(i, j), cmp(i + 1, j), a), mixf(cmp(i, j + 1), cmp(i + 1, j + 1), a), b);
+}
+
+LH_DEV float pow5(float x) {
+    const float x2 = x * x;
+    return (x2 * x2) * x;
+}
+LH_DEV float F_Schlick1(float u, float f0, float f90) { return f0 + (f90 - f0) * pow5(1.0f - u); }
+LH_DEV float Fd_Burley(float NoV, float NoL, float LoH, float roughness) {
+    const float f90 = 0.5f + ((2.0f * roughness) * LoH) * LoH;
+    return (F_Schlick1(NoL, 1.0f, f90) * F_Schlick1(NoV, 1.0f, f90)) * (1.0f / kPi);
+}
+LH_DEV float D_GGX(float NoH, float a) {
+    const float a2 = a * a;
+    const float f = (NoH * a2 - NoH) * NoH + 1.0f;
+    return a2 / ((kPi * f) * f);
+}
+LH_DEV float V_SmithGGXCorrelated(float NoV, float NoL, float a) {
+    const float a2 = a * a;
+    const float GGXL = NoV * sqrtf((-NoL * a2 + NoL) * NoL + a2);
+    const float GGXV = NoL * sqrtf((-NoV * a2 + NoV) * NoV + a2);
+    return 0.5f / (GGXV + GGXL);
+}
+
+// BRDF (src/shaders/common/pbr.glsl:44-96).
+LH_DEV f3 brdf(f3 v, f3 l, f3 n, float pr, float metallic, f3 base, bool thin) {
+    const f3 h = normalize3(add(v, l));
+    const f3 diffuse = mul(base, 1.0f - metallic);
+    const float refl = ((1.5f - 1.0f) / (1.5f + 1.0f)) * ((1.5f - 1.0f) / (1.5f + 1.0f));
+    const float d0 = refl * (1.0f - metallic);
+    const f3 f0 = add(f3{d0, d0, d0}, mul(base, metallic));
+    const float NoV = clampf(dot3(n, v), 1e-5f, 1.0f);
+    const float NoL = clampf(dot3(n, l), 0.0f, 1.0f);
+    const float NoH = clampf(dot3(n, h), 0.0f, 1.0f);
+    const float LoH = clampf(dot3(l, h), 0.0f, 1.0f);
+    const float a = pr * pr;
+    const float D = D_GGX(NoH, a);
+    const float p = pow5(1.0f - LoH);
+    const f3 F{f0.x + (1.0f - f0.x) * p, f0.y + (1.0f - f0.y) * p, f0.z + (1.0f - f0.z) * p};
+    const float V = V_SmithGGXCorrelated(NoV, NoL, a);
+    const f3 Fr = mul(F, D * V);
+    const f3 Fd = mul(diffuse, Fd_Burley(NoV, NoL, LoH, a));
+    f3 irr = mul(add(Fd, Fr), NoL);
+    irr = f3{fmaxf(irr.x, 0.0f), fmaxf(irr.y, 0.0f), fmaxf(irr.z, 0.0f)};
+    if (thin) {
+        const f3 nn = neg(n);
+        const float rNoV = clampf(dot3(nn, v), 1e-5f, 1.0f), rNoL = clampf(dot3(nn, l), 0.0f, 1.0f);
+        irr = add(irr, mul(diffuse, Fd_Burley(rNoV, rNoL, LoH, a) * rNoL));
+    }
+    return {isnan(irr.x) ? 0.0f : irr.x, isnan(irr.y) ? 0.0f : irr.y, isnan(irr.z) ? 0.0f : irr.z};
+}
+
+// Decoded geometry of one visibility-buffer sample.
+struct Surface {
+    uint32_t instance, mesh_info, vi[3];
+    f3 local[3];  // mesh-space positions
+    f3 pos[3];    // world positions
+    f4 clip[3];
+};
+
+LH_DEV bool decode(const lh_scene& s, const lh_view& v, uint64_t key, Surface& sf) {
+    if (~uint32_t(key >> 32) == 0u) return false;  // depth 0.0: sky
+    const uint32_t mid = uint32_t(key >> LH_KEY_TRI_BITS) & (LH_MAX_SCENE_MESHLETS - 1);
+    const uint32_t tri = uint32_t(key) & ((1u << LH_KEY_TRI_BITS) - 1);
+    const MeshletReference ref = s.meshlet_table[mid];
+    const Instance& in = s.instances[ref.instance_index];
+    const MeshInfo& mi = mesh_info_at(in.mesh_info_address);
+    const Meshlet& ml = ptr<Meshlet>(mi.meshlets)[ref.meshlet_index];
+    const uint8_t* micro = ptr<uint8_t>(mi.micro_indices) + ml.triangle_offset + 3 * tri;
+    sf.instance = ref.instance_index;
+    sf.mesh_info = uint32_t((in.mesh_info_address - uint64_t(s.mesh_infos)) / sizeof(MeshInfo));
+    for (int j = 0; j < 3; j++) {
+        sf.vi[j] = load_index(mi, ml.index_offset + micro[j]);
+        sf.local[j] = load_position(mi, sf.vi[j]);
+        sf.pos[j] = mat4_point(in.transform, sf.local[j]);
+        sf.clip[j] = mat4_mul_vec4(v.uniforms->combined_perspective_view, f4{sf.pos[j].x, sf.pos[j].y, sf.pos[j].z, 1.0f});
+    }
+    return true;
+}
+
+LH_DEV f3 resolve_pixel(const lh_scene& s, const lh_view& v, uint32_t x, uint32_t y) {
+    const Uniforms& u = *v.uniforms;
+    const MiscStorage& misc = *v.misc;
+    const f2 win{float(u.window_size.x), float(u.window_size.y)};
+    const f2 ndc{float(x) / win.x * 2.0f - 1.0f, float(y) / win.y * 2.0f - 1.0f};  // pixel corner
+    Surface sf;
+    if (!decode(s, v, v.visbuffer[size_t(y) * v.width + x], sf)) return {kSkyR, kSkyG, kSkyB};
+    const Instance& in = s.instances[sf.instance];
+    const MeshInfo& mi = s.mesh_infos[sf.mesh_info];
+    const BaryDeriv bary = calc_full_bary(sf.clip[0], sf.clip[1], sf.clip[2], ndc, win);
+    const Interp3 world = interp3(bary, sf.pos[0], sf.pos[1], sf.pos[2]);
+    const f2 ts{mi.texture_scale.x, mi.texture_scale.y};
+    f2 uvs[3];
+    for (int j = 0; j < 3; j++) {
+        const f2 t = load_uv(mi, sf.vi[j]);
+        uvs[j] = {t.x * ts.x, t.y * ts.y};
+    }
+    Interp2 uv = interp2(bary, uvs[0], uvs[1], uvs[2]);
+    uv.v.x += mi.texture_offset.x;
+    uv.v.y += mi.texture_offset.y;
+    const f3 view_vector = normalize3(sub(mk3(u.camera_pos), world.v));
+    f3 nrm = interp3(bary, normalize3(load_normal(mi, sf.vi[0])), normalize3(load_normal(mi, sf.vi[1])),
+                     normalize3(load_normal(mi, sf.vi[2])))
+                 .v;
+    nrm = normalize3(mat3_mul_vec3(in.normal_transform, nrm));
+    const f3 gn = cross3(sub(sf.pos[1], sf.pos[0]), sub(sf.pos[2], sf.pos[0]));
+    if (dot3(gn, view_vector) < 0.0f) nrm = neg(nrm);
+    uint32_t cascade;
+    f4 sc{0, 0, 0, 0};
+    for (cascade = 0; cascade < 4; cascade++) {
+        sc = mat4_mul_vec4(misc.uv_space_shadow_matrices[cascade], f4{world.v.x, world.v.y, world.v.z, 1.0f});
+        if (fabsf(sc.x - 0.5f) < 0.5f && fabsf(sc.y - 0.5f) < 0.5f) break;
+    }
+    const float bias = (mi.flags & MESH_INFO_FLAGS_ALPHA_CLIP) ? 0.000002f : 0.0f;
+    sc = f4{sc.x / sc.w, sc.y / sc.w, sc.z / sc.w, sc.w / sc.w};
+    const uint32_t layer = min(cascade, 3u);
+    float shadow_sum = 0.0f;
+    const float S = float(v.shadow_size);
+    for (int ox = -1; ox <= 1; ox++)
+        for (int oy = -1; oy <= 1; oy++)
+            shadow_sum += shadow_tap(v, layer, f2{sc.x + float(ox) / S, sc.y + float(oy) / S}, sc.z - bias);
+    shadow_sum /= 9.0f;
+    if (sc.z > 1.0f) shadow_sum = 1.0f;
+    const LhMaterialFactors fac = s.factors[sf.mesh_info];
+    const f4 bc = sample_texture(s, mi.base_color_texture_index, uv);
+    const f3 base = mul(f3{bc.x, bc.y, bc.z}, mk3(mi.base_color_factor));
+    const f4 mr = sample_texture(s, mi.metallic_roughness_texture_index, uv);
+    const float roughness = mr.y * fac.roughness_factor;
+    const float metallic = mr.z * fac.metallic_factor;
+    if (mi.normal_texture_index != UNUSED_TEXTURE_INDEX) {
+        const f4 mn = sample_texture(s, mi.normal_texture_index, uv);
+        const f3 map{mn.x * 255.0f / 127.0f - 128.0f / 127.0f, mn.y * 255.0f / 127.0f - 128.0f / 127.0f,
+                     mn.z * 255.0f / 127.0f - 128.0f / 127.0f};
+        const f3 dpy_perp = cross3(world.dy, nrm), dpx_perp = cross3(nrm, world.dx);
+        f3 t = add(mul(dpy_perp, uv.dx.x), mul(dpx_perp, uv.dy.x));
+        f3 b = add(mul(dpy_perp, uv.dx.y), mul(dpx_perp, uv.dy.y));
+        const float invmax = 1.0f / sqrtf(fmaxf(dot3(t, t), dot3(b, b)));
+        t = mul(t, invmax);
+        b = mul(b, invmax);
+        nrm = normalize3(f3{(t.x * map.x + b.x * map.y) + nrm.x * map.z, (t.y * map.x + b.y * map.y) + nrm.y * map.z,
+                            (t.z * map.x + b.z * map.y) + nrm.z * map.z});
+    }
+    const float L = 0.025f;
+    const float shv = (((((((L + 0.0f * nrm.x) + L * nrm.y) + 0.0f * nrm.z) + 0.0f * (nrm.x * nrm.z)) +
+                         0.0f * (nrm.z * nrm.y)) +
+                        0.0f * (nrm.y * nrm.x)) +
+                       0.0f * ((3.0f * nrm.z) * nrm.z - 1.0f)) +
+                      0.0f * (nrm.x * nrm.x - nrm.y * nrm.y);
+    const f3 ambient = mul(mul(base, 1.0f - metallic), shv);
+    const f3 sun = mul(mul(brdf(view_vector, mk3(u.sun_dir), nrm, roughness, metallic, base,
+                                (mi.flags & MESH_INFO_FLAGS_ALPHA_CLIP) != 0),
+                           kPi * shadow_sum),
+                       mk3(u.sun_intensity));
+    return add(sun, ambient);
+}
+
+LH_DEV float srgb_oetf(float x) { return x <= 0.003130f ? x * 12.92f : 1.055f * powf(x, 1.0f / 2.4f) - 0.055f; }
+LH_DEV uint32_t encode8(float x) {
+    x = fmaxf(x, 0.0f);
+    const float t = clampf(srgb_oetf(x / (x + 1.0f)), 0.0f, 1.0f);
+    return uint32_t(t * 255.0f + 0.5f);
+}
+LH_DEV uint32_t tonemap_pixel(f3 c) { return encode8(c.z) | (encode8(c.y) << 8) | (encode8(c.x) << 16) | (255u << 24); }
+
+}  // namespace shading

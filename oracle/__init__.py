@@ -1,0 +1,66 @@
+"""ctypes binding of the scalar oracle (oracle/liblh_oracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py as the checker.  The product path (liblh.so)
+never loads it.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+_HERE = Path(__file__).resolve().parent
+_lib = None
+
+_VIEW_FNS = ["orc_view_reset", "orc_cull_instances", "orc_raster_view", "orc_depth_bounds", "orc_fit_cascades",
+             "orc_cull_instances_shadow", "orc_resolve", "orc_frame"]
+
+
+def lib() -> C.CDLL:
+    global _lib
+    if _lib is None:
+        path = _HERE / "liblh_oracle.so"
+        if not path.exists():
+            from . import build  # noqa: PLC0415
+
+            build.build()
+        L = C.CDLL(str(path))
+        for name in _VIEW_FNS:
+            getattr(L, name).restype = C.c_int32
+            getattr(L, name).argtypes = [C.c_void_p, C.c_void_p]
+        L.orc_cull_meshlets.restype = C.c_int32
+        L.orc_cull_meshlets.argtypes = [C.c_void_p, C.c_void_p, C.c_uint32]
+        L.orc_raster_shadow.restype = C.c_int32
+        L.orc_raster_shadow.argtypes = [C.c_void_p, C.c_void_p, C.c_uint32]
+        L.orc_tonemap.restype = C.c_int32
+        L.orc_tonemap.argtypes = [C.c_void_p]
+        L.orc_raster_clip_triangle.restype = C.c_uint64
+        L.orc_raster_clip_triangle.argtypes = [C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32,
+                                               C.c_void_p, C.c_void_p]
+        L.orc_cull_sphere.restype = C.c_int
+        L.orc_cull_sphere.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
+        L.orc_cull_sphere_shadow.restype = C.c_int
+        L.orc_cull_sphere_shadow.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32]
+        L.orc_cull_cone.restype = C.c_int
+        L.orc_cull_cone.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int]
+        L.orc_calc_full_bary.restype = None
+        L.orc_calc_full_bary.argtypes = [C.c_void_p] * 6
+        L.orc_brdf.restype = None
+        L.orc_brdf.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_float, C.c_float, C.c_void_p, C.c_int,
+                               C.c_void_p]
+        L.orc_srgb_oetf.restype = C.c_float
+        L.orc_srgb_oetf.argtypes = [C.c_float]
+        if hasattr(L, "orc64_resolve_adjoint"):
+            L.orc64_resolve_adjoint.restype = C.c_int32
+        _lib = L
+    return _lib
+
+
+def check(status: int, what: str = "oracle") -> None:
+    if status not in (0, 2):
+        raise RuntimeError(f"{what} failed with status {status}")
+
+
+def frame(scene, view) -> int:
+    """Runs the whole frame on the CPU: scene/view are SceneData/ViewData in HostMemory."""
+    return lib().orc_frame(scene.address, view.address)

@@ -1,0 +1,122 @@
+"""Parity checks between the HIP path and the oracle.  TEST INFRASTRUCTURE ONLY.
+
+Bars (from the drop-in contract):
+  - visible instance sets and visible meshlet sets: equal as sorted sets
+    (append order is atomic arrival order on the GPU, as in the reference);
+  - prefix-sum sectors: per-slot invariants of an inclusive append;
+  - visibility-buffer keys, shadow-map depth bits, depth bounds and cascade
+    matrices: bit-exact;
+  - FP32 radiance: within 1e-3 relative; tonemapped BGRA8: within 1 step.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from lighthugger_amd import abi
+from lighthugger_amd.frame import HostMemory, SceneData, ViewConfig, ViewData, draw_keys
+
+from . import lib as oracle_lib
+
+RADIANCE_RTOL = 1e-3
+RADIANCE_ATOL = 1e-6
+
+
+def snapshot(view: ViewData) -> dict:
+    """Every output of a view, copied to host memory."""
+    out = {"counters": view.counters(), "misc": view.misc(), "visbuffer": view.visbuffer(),
+           "shadow": view.shadow_maps(), "radiance": view.radiance()}
+    if view.cfg.ldr:
+        out["ldr"] = view.ldr()
+    for p in range(abi.NUM_PASSES):
+        out[f"prefix{p}"] = view.prefix(p)
+        out[f"draws{p}"] = view.draws(p)
+    return out
+
+
+def oracle_view(host_scene, cfg: ViewConfig, uniforms: np.ndarray):
+    mem = HostMemory()
+    sd = SceneData(host_scene, mem)
+    vd = ViewData(cfg, sd, mem)
+    vd.set_uniforms(uniforms, sd)
+    return sd, vd
+
+
+def oracle_frame(host_scene, cfg: ViewConfig, uniforms: np.ndarray) -> dict:
+    sd, vd = oracle_view(host_scene, cfg, uniforms)
+    st = oracle_lib().orc_frame(sd.address, vd.address)
+    if st not in (abi.LH_OK, abi.LH_ERR_CAPACITY):
+        raise RuntimeError(f"oracle frame failed: {st}")
+    return snapshot(vd)
+
+
+def check_prefix_invariants(count: int, total: int, vals: np.ndarray, num_meshlets: np.ndarray) -> None:
+    """Slot-ordered sums of an inclusive append: sum[k] - value(index[k]) == sum[k-1]."""
+    assert len(vals) == count, (len(vals), count)
+    if count == 0:
+        assert total == 0, "empty sector with a non-zero total"
+        return
+    v = num_meshlets[vals["index"].astype(np.int64)].astype(np.int64)
+    s = vals["sum"].astype(np.int64)
+    prev = np.concatenate([[0], s[:-1]])
+    assert np.array_equal(s - v, prev), "prefix sums are not a consistent inclusive scan"
+    assert int(s[-1] if len(s) else 0) == total, "sector total != last inclusive sum"
+    assert len(np.unique(vals["index"])) == len(vals), "an instance was appended twice"
+
+
+def instance_meshlet_counts(host_scene) -> np.ndarray:
+    mi = host_scene.instances["mesh_info_address"].astype(np.int64)
+    return host_scene.mesh_infos["num_meshlets"].astype(np.int64)[mi]
+
+
+def compare_frames(gpu: dict, orc: dict, host_scene, *, check_radiance=True) -> dict:
+    """Asserts parity of every pass; returns a small report."""
+    nm = instance_meshlet_counts(host_scene)
+    report = {}
+    for p in range(abi.NUM_PASSES):
+        gc, gt, gv = gpu[f"prefix{p}"]
+        oc, ot, ov = orc[f"prefix{p}"]
+        check_prefix_invariants(gc, gt, gv, nm)
+        assert (gc, gt) == (oc, ot), f"pass {p}: prefix count/total {gc, gt} != oracle {oc, ot}"
+        assert np.array_equal(np.sort(gv["index"]), np.sort(ov["index"])), f"pass {p}: visible instance set differs"
+        gd, od = draw_keys(gpu[f"draws{p}"]), draw_keys(orc[f"draws{p}"])
+        assert np.array_equal(gd, od), f"pass {p}: visible meshlet set differs ({len(gd)} vs {len(od)})"
+        report[f"pass{p}"] = {"instances": gc, "meshlets": len(gd)}
+    assert gpu["counters"][abi.CNT_OVERFLOW] == 0 and orc["counters"][abi.CNT_OVERFLOW] == 0, "overflow"
+    gm, om = gpu["misc"], orc["misc"]
+    assert (int(gm["min_depth"]), int(gm["max_depth"])) == (int(om["min_depth"]), int(om["max_depth"])), \
+        "depth bounds differ"
+    for f in ("shadow_matrices", "uv_space_shadow_matrices", "shadow_view_matrices", "shadow_sphere_radii"):
+        a, b = np.asarray(gm[f]).view(np.uint32), np.asarray(om[f]).view(np.uint32)
+        nan = np.isnan(np.asarray(om[f]))
+        assert np.array_equal(a[~nan], b[~nan]) and np.array_equal(np.isnan(np.asarray(gm[f])), nan), \
+            f"misc.{f} not bit-exact"
+    gv, ov = gpu["visbuffer"], orc["visbuffer"]
+    diff = gv != ov
+    assert not diff.any(), f"visibility buffer: {int(diff.sum())} of {diff.size} keys differ"
+    report["covered_pixels"] = int((ov != abi.LH_VISBUFFER_CLEAR).sum())
+    gs, os_ = gpu["shadow"], orc["shadow"]
+    sdiff = gs != os_
+    assert not sdiff.any(), f"shadow maps: {int(sdiff.sum())} texels differ"
+    report["shadow_texels"] = [int((os_[c] != abi.LH_SHADOW_CLEAR).sum()) for c in range(abi.NUM_CASCADES)]
+    if check_radiance:
+        report.update(compare_radiance(gpu["radiance"], orc["radiance"]))
+        if "ldr" in gpu and "ldr" in orc:
+            report.update(compare_ldr(gpu["ldr"], orc["ldr"]))
+    return report
+
+
+def compare_radiance(g: np.ndarray, o: np.ndarray) -> dict:
+    g3, o3 = g[..., :3].astype(np.float64), o[..., :3].astype(np.float64)
+    err = np.abs(g3 - o3)
+    bad = err > RADIANCE_RTOL * np.abs(o3) + RADIANCE_ATOL
+    rel = err / np.maximum(np.abs(o3), 1e-30)
+    assert not bad.any(), f"radiance: {int(bad.sum())} channels beyond 1e-3 relative (max rel {rel.max():.3e})"
+    return {"radiance_max_rel": float(rel.max()), "radiance_bitexact_frac": float((err == 0).mean())}
+
+
+def compare_ldr(g: np.ndarray, o: np.ndarray) -> dict:
+    d = 0
+    for sh in (0, 8, 16, 24):
+        d = max(d, int(np.abs(((g >> sh) & 255).astype(np.int32) - ((o >> sh) & 255).astype(np.int32)).max()))
+    assert d <= 1, f"tonemapped output differs by {d} steps"
+    return {"ldr_max_step": d}

@@ -1,0 +1,965 @@
+// Scalar C++ oracle of the GPU-driven frame.  TEST INFRASTRUCTURE ONLY (see
+// oracle.h): the checker for the HIP kernels, never part of the product path.
+//
+// Arithmetic conventions (shared with the kernels by definition, not by code):
+//   dot3(a, b)      = (a.x*b.x + a.y*b.y) + a.z*b.z
+//   length(v)       = sqrt(dot3(v, v)); normalize(v) = (v.x/l, v.y/l, v.z/l)
+//   (M*v)[r]        = ((M[0][r]*v.x + M[1][r]*v.y) + M[2][r]*v.z) + M[3][r]*v.w
+//   (A*B)[c]        = A * B[c]
+//   mix(x, y, a)    = x*(1-a) + y*a
+// compiled with -ffp-contract=off; no transcendental function feeds a
+// bit-exact result (powers of the cascade split are integer powers).
+#include "oracle.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+namespace {
+
+// ------------------------------------------------------------------ math
+
+struct F2 {
+    float x, y;
+};
+struct F3 {
+    float x, y, z;
+};
+struct F4 {
+    float x, y, z, w;
+};
+
+inline F3 f3(const LhVec3& v) { return {v.x, v.y, v.z}; }
+inline F4 f4(const LhVec4& v) { return {v.x, v.y, v.z, v.w}; }
+inline F3 add(F3 a, F3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+inline F3 sub(F3 a, F3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+inline F3 mul(F3 a, float s) { return {a.x * s, a.y * s, a.z * s}; }
+inline F3 mul(F3 a, F3 b) { return {a.x * b.x, a.y * b.y, a.z * b.z}; }
+inline F3 neg(F3 a) { return {-a.x, -a.y, -a.z}; }
+inline float dot3(F3 a, F3 b) { return (a.x * b.x + a.y * b.y) + a.z * b.z; }
+inline float length3(F3 v) { return std::sqrt(dot3(v, v)); }
+inline F3 normalize3(F3 v) {
+    float l = length3(v);
+    return {v.x / l, v.y / l, v.z / l};
+}
+inline F3 cross3(F3 a, F3 b) { return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x}; }
+inline float mixf(float x, float y, float a) { return x * (1.0f - a) + y * a; }
+inline float clampf(float v, float lo, float hi) { return std::min(std::max(v, lo), hi); }
+
+inline float mrow(const LhVec4& c, int r) { return r == 0 ? c.x : r == 1 ? c.y : r == 2 ? c.z : c.w; }
+
+inline F4 mat4_mul_vec4(const LhMat4& m, F4 v) {
+    float o[4];
+    for (int r = 0; r < 4; r++)
+        o[r] = ((mrow(m.c[0], r) * v.x + mrow(m.c[1], r) * v.y) + mrow(m.c[2], r) * v.z) + mrow(m.c[3], r) * v.w;
+    return {o[0], o[1], o[2], o[3]};
+}
+inline F3 mat4_point(const LhMat4& m, F3 p) {
+    F4 r = mat4_mul_vec4(m, F4{p.x, p.y, p.z, 1.0f});
+    return {r.x, r.y, r.z};
+}
+inline F3 mat3_mul_vec3(const LhMat3& m, F3 v) {
+    const LhVec3* c = m.c;
+    return {(c[0].x * v.x + c[1].x * v.y) + c[2].x * v.z, (c[0].y * v.x + c[1].y * v.y) + c[2].y * v.z,
+            (c[0].z * v.x + c[1].z * v.y) + c[2].z * v.z};
+}
+inline LhMat4 mat4_mul(const LhMat4& a, const LhMat4& b) {
+    LhMat4 r;
+    for (int c = 0; c < 4; c++) {
+        F4 v = mat4_mul_vec4(a, f4(b.c[c]));
+        r.c[c] = LhVec4{v.x, v.y, v.z, v.w};
+    }
+    return r;
+}
+inline uint32_t fbits(float f) {
+    uint32_t u;
+    std::memcpy(&u, &f, 4);
+    return u;
+}
+inline float bitsf(uint32_t u) {
+    float f;
+    std::memcpy(&f, &u, 4);
+    return f;
+}
+inline float near_plane(const Uniforms* u) { return u->perspective.c[3].z; }
+
+// Integer power for the cascade split (pow(0.5, p) at
+// src/shaders/compute/generate_shadow_matrices.glsl:121-130); falls back to
+// powf for a non-integral exponent.
+float split_pow(float base, float p) {
+    if (p >= 0.0f && p <= 64.0f && std::floor(p) == p) {
+        float r = 1.0f;
+        for (int i = 0; i < int(p); i++) r = r * base;
+        return r;
+    }
+    return std::pow(base, p);
+}
+
+template <class T>
+const T* at(uint64_t addr) {
+    return reinterpret_cast<const T*>(static_cast<uintptr_t>(addr));
+}
+
+// ------------------------------------------------------- vertex fetching
+
+// load_index (src/shaders/common/util.glsl:3-9).
+uint32_t load_index(const MeshInfo& mi, uint32_t i) {
+    if (mi.flags & MESH_INFO_FLAGS_32_BIT_INDICES) return at<uint32_t>(mi.indices)[i];
+    return at<uint16_t>(mi.indices)[i];
+}
+// QuantizedPositionBuffer (src/shaders/common/buffer_references.glsl:33-36):
+// u16x3 converted to float; FP32 positions with MESH_INFO_FLAGS_FP32_VERTICES.
+F3 load_position(const MeshInfo& mi, uint32_t v) {
+    if (mi.flags & MESH_INFO_FLAGS_FP32_VERTICES) {
+        const float* p = at<float>(mi.positions) + 3 * size_t(v);
+        return {p[0], p[1], p[2]};
+    }
+    const uint16_t* p = at<uint16_t>(mi.positions) + 3 * size_t(v);
+    return {float(p[0]), float(p[1]), float(p[2])};
+}
+// QuanitizedNormals (buffer_references.glsl:43-45): i8x3 -> float3.
+F3 load_normal(const MeshInfo& mi, uint32_t v) {
+    if (mi.flags & MESH_INFO_FLAGS_FP32_VERTICES) {
+        const float* p = at<float>(mi.normals) + 3 * size_t(v);
+        return {p[0], p[1], p[2]};
+    }
+    const int8_t* p = at<int8_t>(mi.normals) + 3 * size_t(v);
+    return {float(p[0]), float(p[1]), float(p[2])};
+}
+// QuanitizedUvs (buffer_references.glsl:39-41): u16x2 -> float2.
+F2 load_uv(const MeshInfo& mi, uint32_t v) {
+    if (mi.flags & MESH_INFO_FLAGS_FP32_VERTICES) {
+        const float* p = at<float>(mi.uvs) + 2 * size_t(v);
+        return {p[0], p[1]};
+    }
+    const uint16_t* p = at<uint16_t>(mi.uvs) + 2 * size_t(v);
+    return {float(p[0]), float(p[1])};
+}
+
+const MeshInfo& mesh_info_of(const Instance& in) { return *at<MeshInfo>(in.mesh_info_address); }
+const Meshlet& meshlet_of(const MeshInfo& mi, uint32_t k) { return at<Meshlet>(mi.meshlets)[k]; }
+// Corner j of triangle k: micro index -> remap -> vertex index
+// (load_vertex_data, src/shaders/rasterization.glsl:12-34).
+uint32_t corner_vertex(const MeshInfo& mi, const Meshlet& ml, uint32_t k, uint32_t j) {
+    uint8_t m = at<uint8_t>(mi.micro_indices)[size_t(ml.triangle_offset) + 3 * k + j];
+    return load_index(mi, ml.index_offset + m);
+}
+
+// ------------------------------------------------------------- culling
+
+// World-space sphere of a (mesh-space) sphere under the instance transform
+// (src/shaders/common/culling.glsl:4-17).
+void world_sphere(const Instance& in, const float* s, F3& centre, float& radius) {
+    centre = mat4_point(in.transform, F3{s[0], s[1], s[2]});
+    const LhMat4& t = in.transform;
+    float sx = length3(F3{t.c[0].x, t.c[0].y, t.c[0].z});
+    float sy = length3(F3{t.c[1].x, t.c[1].y, t.c[1].z});
+    float sz = length3(F3{t.c[2].x, t.c[2].y, t.c[2].z});
+    radius = s[3] * std::max(std::max(sx, sy), sz);
+}
+
+// cull_bounding_sphere (src/shaders/common/culling.glsl:1-46).
+bool cull_sphere(const Instance& in, const float* s, const Uniforms& u) {
+    F3 c;
+    float r;
+    world_sphere(in, s, c, r);
+    F3 v = mat4_point(u.initial_view, c);
+    v.z = -v.z;
+    bool visible = v.z + r > near_plane(&u);
+    const LhMat4& p = u.perspective;
+    // rows 3, 0, 1 of the perspective matrix (transpose(P)[k])
+    F3 row3{p.c[0].w, p.c[1].w, p.c[2].w}, row0{p.c[0].x, p.c[1].x, p.c[2].x}, row1{p.c[0].y, p.c[1].y, p.c[2].y};
+    F3 fx = normalize3(add(row3, row0));
+    F3 fy = normalize3(add(row3, row1));
+    visible = visible && (v.z * fx.z + std::fabs(v.x) * fx.x < r);
+    visible = visible && (v.z * fy.z - std::fabs(v.y) * fy.y < r);
+    return !visible;
+}
+
+// fits_partially_inside / fits_entirely_inside (culling.glsl:48-62).
+bool fits_partially(F2 p, float r, float size) { return (std::fabs(p.x) - r < size) && (std::fabs(p.y) - r < size); }
+bool fits_entirely(F2 p, float r, float size) { return (std::fabs(p.x) + r < size) && (std::fabs(p.y) + r < size); }
+F2 shadow_view_xy(const MiscStorage& m, uint32_t c, F3 w) {
+    F3 v = mat4_point(m.shadow_view_matrices[c], w);
+    return {v.x, v.y};
+}
+
+// cull_bounding_sphere_shadows (culling.glsl:82-135).
+bool cull_sphere_shadow(const Instance& in, const float* s, const MiscStorage& m, uint32_t c) {
+    F3 w;
+    float r;
+    world_sphere(in, s, w, r);
+    if (!fits_partially(shadow_view_xy(m, c, w), r, m.shadow_sphere_radii[c])) return true;
+    for (uint32_t k = 0; k < c; k++)
+        if (fits_entirely(shadow_view_xy(m, k, w), r, m.shadow_sphere_radii[k])) return true;
+    return false;
+}
+
+// cull_cone_perspective / cull_cone_orthographic (culling.glsl:137-151).
+bool cull_cone(const Instance& in, const Meshlet& ml, const Uniforms& u, bool ortho) {
+    F3 axis = normalize3(mat3_mul_vec3(in.normal_transform, f3(ml.cone_axis)));
+    if (ortho) return dot3(f3(u.sun_dir), axis) >= ml.cone_cutoff;
+    F3 apex = mat4_point(in.transform, f3(ml.cone_apex));
+    return dot3(normalize3(sub(apex, f3(u.camera_pos))), normalize3(axis)) >= ml.cone_cutoff;
+}
+
+// ---------------------------------------------------------- prefix sum
+
+uint64_t* sector(const lh_view& v, uint32_t pass) { return v.prefix_sums + size_t(pass) * (1 + v.instance_capacity); }
+
+// prefix_sum_inclusive_append (src/shaders/common/prefix_sum.glsl:17-30).
+void prefix_append(const lh_view& v, uint32_t pass, uint32_t index, uint32_t value) {
+    uint64_t* s = sector(v, pass);
+    uint64_t old = s[0];
+    s[0] = old + (uint64_t(value) | (uint64_t(1) << 32));
+    uint32_t slot = uint32_t(old >> 32);
+    if (slot >= v.instance_capacity) {
+        v.counters[LH_CNT_OVERFLOW] |= LH_OVERFLOW_INSTANCES;
+        return;
+    }
+    PrefixSumValue* vals = reinterpret_cast<PrefixSumValue*>(s + 1);
+    vals[slot].index = index;
+    vals[slot].sum = uint32_t(old) + value;
+}
+
+// prefix_sum_binary_search (prefix_sum.glsl:33-46), std::upper_bound on sum.
+bool prefix_search(const lh_view& v, uint32_t pass, uint32_t target, PrefixSumValue& out) {
+    const uint64_t* s = sector(v, pass);
+    uint32_t count = std::min(uint32_t(s[0] >> 32), v.instance_capacity);
+    const PrefixSumValue* vals = reinterpret_cast<const PrefixSumValue*>(s + 1);
+    uint32_t first = 0;
+    while (count > 0) {
+        uint32_t step = count / 2, cur = first + step;
+        if (target >= vals[cur].sum) {
+            first = cur + 1;
+            count -= step + 1;
+        } else {
+            count = step;
+        }
+    }
+    uint32_t n = std::min(uint32_t(s[0] >> 32), v.instance_capacity);
+    if (first >= n) return false;
+    out = vals[first];
+    return true;
+}
+
+// --------------------------------------------------------------- raster
+
+constexpr float kGuard = 4.0f;  // guard band |x|, |y| <= 4 w
+
+struct CV {
+    float x, y, z, w;
+};
+struct SV {
+    int32_t X, Y;
+    float z;
+};
+
+float plane_dist(const CV& v, int k) {
+    switch (k) {
+        case 0: return v.w - v.z;        // z <= w
+        case 1: return v.z;              // z >= 0
+        case 2: return kGuard * v.w - v.x;
+        case 3: return kGuard * v.w + v.x;
+        case 4: return kGuard * v.w - v.y;
+        default: return kGuard * v.w + v.y;
+    }
+}
+uint32_t outcode(const CV& v) {
+    uint32_t c = 0;
+    for (int k = 0; k < 6; k++)
+        if (!(plane_dist(v, k) >= 0.0f)) c |= 1u << k;
+    return c;
+}
+bool finite4(const CV& v) { return std::isfinite(v.x) && std::isfinite(v.y) && std::isfinite(v.z) && std::isfinite(v.w); }
+
+// Intersection of the segment (in -> out) with plane k, always computed from
+// the inside end so a shared edge clips to the same point in both triangles.
+CV clip_point(const CV& in, const CV& out, float din, float dout) {
+    float t = din / (din - dout);
+    return {in.x + (out.x - in.x) * t, in.y + (out.y - in.y) * t, in.z + (out.z - in.z) * t,
+            in.w + (out.w - in.w) * t};
+}
+
+// Sutherland-Hodgman against the six planes; returns the vertex count (<= 9).
+int clip_polygon(CV* poly, int n) {
+    CV tmp[12];
+    for (int k = 0; k < 6 && n >= 3; k++) {
+        int m = 0;
+        for (int i = 0; i < n; i++) {
+            const CV& a = poly[i];
+            const CV& b = poly[(i + 1) % n];
+            float da = plane_dist(a, k), db = plane_dist(b, k);
+            bool ia = da >= 0.0f, ib = db >= 0.0f;
+            if (ia) tmp[m++] = a;
+            if (ia != ib) tmp[m++] = ia ? clip_point(a, b, da, db) : clip_point(b, a, db, da);
+        }
+        for (int i = 0; i < m; i++) poly[i] = tmp[i];
+        n = m;
+    }
+    return n >= 3 ? n : 0;
+}
+
+// Viewport transform (x_f = (W/2) x_ndc + W/2) and 8-bit subpixel snapping.
+bool project(const CV& c, uint32_t W, uint32_t H, SV& o) {
+    if (!(c.w > 0.0f)) return false;
+    float xn = c.x / c.w, yn = c.y / c.w, zn = c.z / c.w;
+    float hw = 0.5f * float(W), hh = 0.5f * float(H);
+    float xf = xn * hw + hw, yf = yn * hh + hh;
+    o.X = int32_t(std::rint(xf * 256.0f));
+    o.Y = int32_t(std::rint(yf * 256.0f));
+    o.z = zn;
+    return true;
+}
+
+struct RasterTarget {
+    uint32_t W, H, kind;  // kind 0 main (u64 keys), 1 shadow (u32 depth)
+    uint64_t* vis;
+    uint32_t* depth;
+};
+
+inline int64_t edge(int32_t ax, int32_t ay, int32_t bx, int32_t by, int32_t px, int32_t py) {
+    return int64_t(bx - ax) * int64_t(py - ay) - int64_t(by - ay) * int64_t(px - ax);
+}
+inline bool top_left(int32_t ax, int32_t ay, int32_t bx, int32_t by) {
+    int32_t dx = bx - ax, dy = by - ay;
+    return dy < 0 || (dy == 0 && dx > 0);
+}
+
+// Rasterizes one snapped triangle.  Main view keeps front faces (signed area
+// < 0 in y-down framebuffer coordinates = counter-clockwise on screen,
+// src/pipelines.cpp:10-13 with the default VK_FRONT_FACE_COUNTER_CLOCKWISE);
+// shadows keep back faces (cull FRONT, src/pipelines.cpp:15-19).
+uint64_t raster_snapped(SV v0, SV v1, SV v2, const RasterTarget& t, uint32_t id) {
+    int64_t area = edge(v0.X, v0.Y, v1.X, v1.Y, v2.X, v2.Y);
+    if (t.kind == 0 ? area >= 0 : area <= 0) return 0;
+    if (area < 0) {
+        std::swap(v1, v2);
+        area = -area;
+    }
+    int32_t minX = std::min({v0.X, v1.X, v2.X}), maxX = std::max({v0.X, v1.X, v2.X});
+    int32_t minY = std::min({v0.Y, v1.Y, v2.Y}), maxY = std::max({v0.Y, v1.Y, v2.Y});
+    int32_t xlo = std::max((minX + 127) >> 8, 0), xhi = std::min((maxX - 128) >> 8, int32_t(t.W) - 1);
+    int32_t ylo = std::max((minY + 127) >> 8, 0), yhi = std::min((maxY - 128) >> 8, int32_t(t.H) - 1);
+    if (xlo > xhi || ylo > yhi) return 0;
+    bool tl0 = top_left(v1.X, v1.Y, v2.X, v2.Y), tl1 = top_left(v2.X, v2.Y, v0.X, v0.Y),
+         tl2 = top_left(v0.X, v0.Y, v1.X, v1.Y);
+    float inv_area = 1.0f / float(area);
+    float d10 = v1.z - v0.z, d20 = v2.z - v0.z;
+    uint64_t frags = 0;
+    for (int32_t py = ylo; py <= yhi; py++) {
+        int32_t Py = py * 256 + 128;
+        for (int32_t px = xlo; px <= xhi; px++) {
+            int32_t Px = px * 256 + 128;
+            int64_t e0 = edge(v1.X, v1.Y, v2.X, v2.Y, Px, Py);
+            int64_t e1 = edge(v2.X, v2.Y, v0.X, v0.Y, Px, Py);
+            int64_t e2 = edge(v0.X, v0.Y, v1.X, v1.Y, Px, Py);
+            if (!(e0 > 0 || (e0 == 0 && tl0))) continue;
+            if (!(e1 > 0 || (e1 == 0 && tl1))) continue;
+            if (!(e2 > 0 || (e2 == 0 && tl2))) continue;
+            float l1 = float(e1) * inv_area, l2 = float(e2) * inv_area;
+            float z = (v0.z + l1 * d10) + l2 * d20;
+            z = std::min(std::max(z, 0.0f), 1.0f);
+            frags++;
+            size_t p = size_t(py) * t.W + px;
+            if (t.kind == 0) {
+                if (!(z > 0.0f)) continue;  // depth GREATER against the 0.0 clear
+                uint64_t key = (uint64_t(~fbits(z)) << 32) | id;
+                if (key < t.vis[p]) t.vis[p] = key;
+            } else {
+                uint32_t d = fbits(z);
+                if (d < t.depth[p]) t.depth[p] = d;
+            }
+        }
+    }
+    return frags;
+}
+
+uint64_t raster_clip_triangle(const CV* c, const RasterTarget& t, uint32_t id) {
+    if (!finite4(c[0]) || !finite4(c[1]) || !finite4(c[2])) return 0;
+    uint32_t o0 = outcode(c[0]), o1 = outcode(c[1]), o2 = outcode(c[2]);
+    if (o0 & o1 & o2) return 0;
+    if ((o0 | o1 | o2) == 0) {
+        SV s[3];
+        for (int i = 0; i < 3; i++)
+            if (!project(c[i], t.W, t.H, s[i])) return 0;
+        return raster_snapped(s[0], s[1], s[2], t, id);
+    }
+    CV poly[12] = {c[0], c[1], c[2]};
+    int n = clip_polygon(poly, 3);
+    if (n == 0) return 0;
+    SV s[12];
+    for (int i = 0; i < n; i++)
+        if (!project(poly[i], t.W, t.H, s[i])) return 0;
+    uint64_t frags = 0;
+    for (int i = 1; i + 1 < n; i++) frags += raster_snapped(s[0], s[i], s[i + 1], t, id);
+    return frags;
+}
+
+// Rasterizes every triangle of every meshlet in draws[pass]
+// (drawIndirectCount of src/rendering.cpp:235-242 / :405-412).
+void raster_pass(const lh_scene& sc, const lh_view& v, uint32_t pass) {
+    const Uniforms& u = *v.uniforms;
+    uint32_t count = std::min(v.counters[LH_CNT_DRAWS + pass], v.draw_capacity);
+    const MeshletReference* draws = v.draws + size_t(pass) * v.draw_capacity;
+    RasterTarget t;
+    const LhMat4* m;
+    if (pass == 0) {
+        t = RasterTarget{v.width, v.height, 0, v.visbuffer, nullptr};
+        m = &u.combined_perspective_view;
+    } else {
+        uint32_t c = pass - 1;
+        t = RasterTarget{v.shadow_size, v.shadow_size, 1, nullptr,
+                         v.shadow_maps + size_t(c) * v.shadow_size * v.shadow_size};
+        m = &v.misc->shadow_matrices[c];
+    }
+    for (uint32_t d = 0; d < count; d++) {
+        const MeshletReference& ref = draws[d];
+        const Instance& in = sc.instances[ref.instance_index];
+        const MeshInfo& mi = mesh_info_of(in);
+        const Meshlet& ml = meshlet_of(mi, ref.meshlet_index);
+        uint32_t mid = sc.instance_meshlet_base[ref.instance_index] + ref.meshlet_index;
+        for (uint32_t k = 0; k < ml.triangle_count; k++) {
+            CV c[3];
+            for (uint32_t j = 0; j < 3; j++) {
+                // calculate_world_pos (util.glsl:11-16) then the view matrix.
+                F3 w = mat4_point(in.transform, load_position(mi, corner_vertex(mi, ml, k, j)));
+                F4 q = mat4_mul_vec4(*m, F4{w.x, w.y, w.z, 1.0f});
+                c[j] = CV{q.x, q.y, q.z, q.w};
+            }
+            raster_clip_triangle(c, t, (mid << LH_KEY_TRI_BITS) | k);
+        }
+    }
+}
+
+// ------------------------------------------------------------- resolve
+
+struct BaryDeriv {
+    F3 lambda, ddx, ddy;
+};
+
+// CalcFullBary (src/shaders/common/vbuffer.glsl:11-63).
+BaryDeriv calc_full_bary(F4 p0, F4 p1, F4 p2, F2 ndc, F2 win) {
+    BaryDeriv r;
+    F3 invW{1.0f / p0.w, 1.0f / p1.w, 1.0f / p2.w};
+    F2 n0{p0.x * invW.x, p0.y * invW.x}, n1{p1.x * invW.y, p1.y * invW.y}, n2{p2.x * invW.z, p2.y * invW.z};
+    F2 c0{n2.x - n1.x, n2.y - n1.y}, c1{n0.x - n1.x, n0.y - n1.y};
+    float invDet = 1.0f / (c0.x * c1.y - c1.x * c0.y);
+    r.ddx = mul(mul(F3{n1.y - n2.y, n2.y - n0.y, n0.y - n1.y}, invDet), invW);
+    r.ddy = mul(mul(F3{n2.x - n1.x, n0.x - n2.x, n1.x - n0.x}, invDet), invW);
+    float ddxSum = (r.ddx.x + r.ddx.y) + r.ddx.z;
+    float ddySum = (r.ddy.x + r.ddy.y) + r.ddy.z;
+    F2 dv{ndc.x - n0.x, ndc.y - n0.y};
+    float interpInvW = (invW.x + dv.x * ddxSum) + dv.y * ddySum;
+    float interpW = 1.0f / interpInvW;
+    r.lambda.x = interpW * ((invW.x + dv.x * r.ddx.x) + dv.y * r.ddy.x);
+    r.lambda.y = interpW * ((0.0f + dv.x * r.ddx.y) + dv.y * r.ddy.y);
+    r.lambda.z = interpW * ((0.0f + dv.x * r.ddx.z) + dv.y * r.ddy.z);
+    float sx = 2.0f / win.x, sy = 2.0f / win.y;
+    r.ddx = mul(r.ddx, sx);
+    r.ddy = mul(r.ddy, sy);
+    ddxSum *= sx;
+    ddySum *= sy;
+    r.ddy = mul(r.ddy, -1.0f);
+    ddySum *= -1.0f;
+    float wdx = 1.0f / (interpInvW + ddxSum), wdy = 1.0f / (interpInvW + ddySum);
+    r.ddx = F3{wdx * (r.lambda.x * interpInvW + r.ddx.x) - r.lambda.x, wdx * (r.lambda.y * interpInvW + r.ddx.y) - r.lambda.y,
+               wdx * (r.lambda.z * interpInvW + r.ddx.z) - r.lambda.z};
+    r.ddy = F3{wdy * (r.lambda.x * interpInvW + r.ddy.x) - r.lambda.x, wdy * (r.lambda.y * interpInvW + r.ddy.y) - r.lambda.y,
+               wdy * (r.lambda.z * interpInvW + r.ddy.z) - r.lambda.z};
+    return r;
+}
+
+// InterpolateWithDeriv (vbuffer.glsl:65-73): (value, d/dx, d/dy).
+F3 interp1(const BaryDeriv& d, float a, float b, float c) {
+    F3 m{a, b, c};
+    return {dot3(m, d.lambda), dot3(m, d.ddx), dot3(m, d.ddy)};
+}
+
+struct Interp3 {
+    F3 v, dx, dy;
+};
+struct Interp2 {
+    F2 v, dx, dy;
+};
+Interp3 interp3(const BaryDeriv& d, F3 a, F3 b, F3 c) {
+    F3 x = interp1(d, a.x, b.x, c.x), y = interp1(d, a.y, b.y, c.y), z = interp1(d, a.z, b.z, c.z);
+    return {{x.x, y.x, z.x}, {x.y, y.y, z.y}, {x.z, y.z, z.z}};
+}
+Interp2 interp2(const BaryDeriv& d, F2 a, F2 b, F2 c) {
+    F3 x = interp1(d, a.x, b.x, c.x), y = interp1(d, a.y, b.y, c.y);
+    return {{x.x, y.x}, {x.y, y.y}, {x.z, y.z}};
+}
+
+// RGBA8 UNORM texel fetch with repeat addressing.
+F4 texel(const LhTexture& t, uint32_t level, int32_t i, int32_t j) {
+    int32_t w = int32_t(std::max(1u, t.width >> level)), h = int32_t(std::max(1u, t.height >> level));
+    i %= w;
+    if (i < 0) i += w;
+    j %= h;
+    if (j < 0) j += h;
+    const uint8_t* p = at<uint8_t>(t.texels) + 4 * (size_t(t.level_offset[level]) + size_t(j) * w + i);
+    return {p[0] / 255.0f, p[1] / 255.0f, p[2] / 255.0f, p[3] / 255.0f};
+}
+F4 mix4(F4 a, F4 b, float t) { return {mixf(a.x, b.x, t), mixf(a.y, b.y, t), mixf(a.z, b.z, t), mixf(a.w, b.w, t)}; }
+F4 bilinear(const LhTexture& t, uint32_t level, F2 uv) {
+    float w = float(std::max(1u, t.width >> level)), h = float(std::max(1u, t.height >> level));
+    float u = uv.x * w - 0.5f, v = uv.y * h - 0.5f;
+    float fu = std::floor(u), fv = std::floor(v);
+    float a = u - fu, b = v - fv;
+    int32_t i = int32_t(fu), j = int32_t(fv);
+    F4 t00 = texel(t, level, i, j), t10 = texel(t, level, i + 1, j), t01 = texel(t, level, i, j + 1),
+       t11 = texel(t, level, i + 1, j + 1);
+    return mix4(mix4(t00, t10, a), mix4(t01, t11, a), b);
+}
+
+// sample_texture (src/shaders/render_geometry.comp:59-66): textureGrad with
+// the repeat sampler, restated as isotropic trilinear filtering with the LOD
+// from the analytic uv derivatives (the reference sampler asks for 16x
+// anisotropy, src/main.cpp:550-560, which is implementation defined).
+// An unused texture index reads as opaque white.
+F4 sample_texture(const lh_scene& sc, uint32_t index, const Interp2& uv) {
+    if (index == UNUSED_TEXTURE_INDEX || index >= sc.num_textures) return {1, 1, 1, 1};
+    const LhTexture& t = sc.textures[index];
+    float W = float(t.width), H = float(t.height);
+    float ax = uv.dx.x * W, ay = uv.dx.y * H, bx = uv.dy.x * W, by = uv.dy.y * H;
+    float rho = std::max(std::sqrt(ax * ax + ay * ay), std::sqrt(bx * bx + by * by));
+    float lambda = rho > 0.0f ? std::log2(rho) : 0.0f;
+    lambda = clampf(lambda, 0.0f, float(t.levels - 1));
+    float d0 = std::floor(lambda), f = lambda - d0;
+    uint32_t l0 = uint32_t(d0), l1 = std::min(l0 + 1, t.levels - 1);
+    F4 c0 = bilinear(t, l0, uv.v);
+    if (f == 0.0f) return c0;
+    return mix4(c0, bilinear(t, l1, uv.v), f);
+}
+
+// One tap of sampler2DArrayShadow with linear filtering, clamp to edge and
+// compare LESS (result 1 where ref < texel), render_geometry.comp:211-218.
+float shadow_tap(const lh_view& v, uint32_t layer, F2 uv, float ref) {
+    const uint32_t S = v.shadow_size;
+    const uint32_t* map = v.shadow_maps + size_t(layer) * S * S;
+    float u = uv.x * float(S) - 0.5f, w = uv.y * float(S) - 0.5f;
+    float fu = std::floor(u), fw = std::floor(w);
+    float a = u - fu, b = w - fw;
+    int32_t i = int32_t(fu), j = int32_t(fw);
+    auto cmp = [&](int32_t x, int32_t y) {
+        x = std::min(std::max(x, 0), int32_t(S) - 1);
+        y = std::min(std::max(y, 0), int32_t(S) - 1);
+        return ref < bitsf(map[size_t(y) * S + x]) ? 1.0f : 0.0f;
+    };
+    return mixf(mixf(cmp(i, j), cmp(i + 1, j), a), mixf(cmp(i, j + 1), cmp(i + 1, j + 1), a), b);
+}
+
+constexpr float kPi = 3.14159265358979323846f;
+inline float pow5(float x) {
+    float x2 = x * x;
+    return (x2 * x2) * x;
+}
+// Filament terms (src/shaders/common/pbr.glsl:1-42).
+float F_Schlick1(float u, float f0, float f90) { return f0 + (f90 - f0) * pow5(1.0f - u); }
+float Fd_Burley(float NoV, float NoL, float LoH, float roughness) {
+    float f90 = 0.5f + ((2.0f * roughness) * LoH) * LoH;
+    float ls = F_Schlick1(NoL, 1.0f, f90), vs = F_Schlick1(NoV, 1.0f, f90);
+    return (ls * vs) * (1.0f / kPi);
+}
+float D_GGX(float NoH, float a) {
+    float a2 = a * a;
+    float f = (NoH * a2 - NoH) * NoH + 1.0f;
+    return a2 / ((kPi * f) * f);
+}
+F3 F_Schlick3(float u, F3 f0) {
+    float p = pow5(1.0f - u);
+    return {f0.x + (1.0f - f0.x) * p, f0.y + (1.0f - f0.y) * p, f0.z + (1.0f - f0.z) * p};
+}
+float V_SmithGGXCorrelated(float NoV, float NoL, float a) {
+    float a2 = a * a;
+    float GGXL = NoV * std::sqrt((-NoL * a2 + NoL) * NoL + a2);
+    float GGXV = NoL * std::sqrt((-NoV * a2 + NoV) * NoV + a2);
+    return 0.5f / (GGXV + GGXL);
+}
+F3 diffuse_color(F3 base, float metallic) { return mul(base, 1.0f - metallic); }
+
+// BRDF (src/shaders/common/pbr.glsl:44-96).
+F3 brdf(F3 v, F3 l, F3 n, float pr, float metallic, F3 base, bool thin) {
+    F3 h = normalize3(add(v, l));
+    F3 diffuse = diffuse_color(base, metallic);
+    const float refl = ((1.5f - 1.0f) / (1.5f + 1.0f)) * ((1.5f - 1.0f) / (1.5f + 1.0f));
+    F3 f0 = add(F3{refl * (1.0f - metallic), refl * (1.0f - metallic), refl * (1.0f - metallic)}, mul(base, metallic));
+    float NoV = clampf(dot3(n, v), 1e-5f, 1.0f);
+    float NoL = clampf(dot3(n, l), 0.0f, 1.0f);
+    float NoH = clampf(dot3(n, h), 0.0f, 1.0f);
+    float LoH = clampf(dot3(l, h), 0.0f, 1.0f);
+    float a = pr * pr;
+    float D = D_GGX(NoH, a);
+    F3 F = F_Schlick3(LoH, f0);
+    float V = V_SmithGGXCorrelated(NoV, NoL, a);
+    F3 Fr = mul(F, D * V);
+    F3 Fd = mul(diffuse, Fd_Burley(NoV, NoL, LoH, a));
+    F3 irr = mul(add(Fd, Fr), NoL);
+    irr = F3{std::max(irr.x, 0.0f), std::max(irr.y, 0.0f), std::max(irr.z, 0.0f)};
+    if (thin) {
+        F3 nn = neg(n);
+        float rNoV = clampf(dot3(nn, v), 1e-5f, 1.0f), rNoL = clampf(dot3(nn, l), 0.0f, 1.0f);
+        irr = add(irr, mul(diffuse, Fd_Burley(rNoV, rNoL, LoH, a) * rNoL));
+    }
+    return {std::isnan(irr.x) ? 0.0f : irr.x, std::isnan(irr.y) ? 0.0f : irr.y, std::isnan(irr.z) ? 0.0f : irr.z};
+}
+
+// Constant sky (the reference samples a skybox cubemap that is absent here,
+// render_geometry.comp:86-101): base colour times 2^3 exposure.
+const F3 kSky{0.35f * 8.0f, 0.50f * 8.0f, 0.80f * 8.0f};
+
+// render_geometry main() (src/shaders/render_geometry.comp:70-266) for one pixel.
+F3 resolve_pixel(const lh_scene& sc, const lh_view& v, uint32_t x, uint32_t y) {
+    const Uniforms& u = *v.uniforms;
+    const MiscStorage& misc = *v.misc;
+    F2 win{float(u.window_size.x), float(u.window_size.y)};
+    F2 ndc{float(x) / win.x * 2.0f - 1.0f, float(y) / win.y * 2.0f - 1.0f};  // pixel corner (:82-84)
+    uint64_t key = v.visbuffer[size_t(y) * v.width + x];
+    if (~uint32_t(key >> 32) == 0u) return kSky;  // depth 0.0
+    uint32_t mid = uint32_t(key >> LH_KEY_TRI_BITS) & (LH_MAX_SCENE_MESHLETS - 1);
+    uint32_t tri = uint32_t(key) & ((1u << LH_KEY_TRI_BITS) - 1);
+    const MeshletReference& ref = sc.meshlet_table[mid];
+    const Instance& in = sc.instances[ref.instance_index];
+    const MeshInfo& mi = mesh_info_of(in);
+    const Meshlet& ml = meshlet_of(mi, ref.meshlet_index);
+    uint32_t vi[3];
+    F3 pos[3];
+    F4 clip[3];
+    for (int j = 0; j < 3; j++) {
+        vi[j] = corner_vertex(mi, ml, tri, j);
+        pos[j] = mat4_point(in.transform, load_position(mi, vi[j]));
+        clip[j] = mat4_mul_vec4(u.combined_perspective_view, F4{pos[j].x, pos[j].y, pos[j].z, 1.0f});
+    }
+    BaryDeriv bary = calc_full_bary(clip[0], clip[1], clip[2], ndc, win);
+    Interp3 world = interp3(bary, pos[0], pos[1], pos[2]);
+    F2 ts{mi.texture_scale.x, mi.texture_scale.y};
+    F2 uvs[3];
+    for (int j = 0; j < 3; j++) {
+        F2 t = load_uv(mi, vi[j]);
+        uvs[j] = {t.x * ts.x, t.y * ts.y};
+    }
+    Interp2 uv = interp2(bary, uvs[0], uvs[1], uvs[2]);
+    uv.v.x += mi.texture_offset.x;
+    uv.v.y += mi.texture_offset.y;
+    F3 view_vector = normalize3(sub(f3(u.camera_pos), world.v));
+    F3 nrm = interp3(bary, normalize3(load_normal(mi, vi[0])), normalize3(load_normal(mi, vi[1])),
+                     normalize3(load_normal(mi, vi[2])))
+                 .v;
+    nrm = normalize3(mat3_mul_vec3(in.normal_transform, nrm));
+    F3 gn = cross3(sub(pos[1], pos[0]), sub(pos[2], pos[0]));
+    if (dot3(gn, view_vector) < 0.0f) nrm = neg(nrm);
+    // Cascade selection (:183-196); the loop may end at 4, which the array
+    // layer clamps to 3.
+    uint32_t cascade;
+    F4 sc4{0, 0, 0, 0};
+    for (cascade = 0; cascade < 4; cascade++) {
+        sc4 = mat4_mul_vec4(misc.uv_space_shadow_matrices[cascade], F4{world.v.x, world.v.y, world.v.z, 1.0f});
+        float ox = sc4.x - 0.5f, oy = sc4.y - 0.5f;
+        if (std::fabs(ox) < 0.5f && std::fabs(oy) < 0.5f) break;
+    }
+    float bias = (mi.flags & MESH_INFO_FLAGS_ALPHA_CLIP) ? 0.000002f : 0.0f;
+    sc4 = F4{sc4.x / sc4.w, sc4.y / sc4.w, sc4.z / sc4.w, sc4.w / sc4.w};
+    uint32_t layer = std::min(cascade, 3u);
+    float shadow_sum = 0.0f;
+    const float S = float(v.shadow_size);
+    for (int ox = -1; ox <= 1; ox++)
+        for (int oy = -1; oy <= 1; oy++) {
+            F2 o{float(ox) / S, float(oy) / S};
+            shadow_sum += shadow_tap(v, layer, F2{sc4.x + o.x, sc4.y + o.y}, sc4.z - bias);
+        }
+    shadow_sum /= 9.0f;
+    if (sc4.z > 1.0f) shadow_sum = 1.0f;
+    // Material (:228-246).
+    const LhMaterialFactors& fac = sc.factors[(in.mesh_info_address - uint64_t(uintptr_t(sc.mesh_infos))) / sizeof(MeshInfo)];
+    F4 bc = sample_texture(sc, mi.base_color_texture_index, uv);
+    F3 base = mul(F3{bc.x, bc.y, bc.z}, f3(mi.base_color_factor));
+    F4 mr = sample_texture(sc, mi.metallic_roughness_texture_index, uv);
+    float roughness = mr.y * fac.roughness_factor;
+    float metallic = mr.z * fac.metallic_factor;
+    if (mi.normal_texture_index != UNUSED_TEXTURE_INDEX) {
+        F4 mn = sample_texture(sc, mi.normal_texture_index, uv);
+        F3 map{mn.x * 255.0f / 127.0f - 128.0f / 127.0f, mn.y * 255.0f / 127.0f - 128.0f / 127.0f,
+               mn.z * 255.0f / 127.0f - 128.0f / 127.0f};
+        // compute_cotangent_frame (:25-38)
+        F3 dpy_perp = cross3(world.dy, nrm), dpx_perp = cross3(nrm, world.dx);
+        F3 t = add(mul(dpy_perp, uv.dx.x), mul(dpx_perp, uv.dy.x));
+        F3 b = add(mul(dpy_perp, uv.dx.y), mul(dpx_perp, uv.dy.y));
+        float invmax = 1.0f / std::sqrt(std::max(dot3(t, t), dot3(b, b)));
+        t = mul(t, invmax);
+        b = mul(b, invmax);
+        nrm = normalize3(F3{(t.x * map.x + b.x * map.y) + nrm.x * map.z, (t.y * map.x + b.y * map.y) + nrm.y * map.z,
+                            (t.z * map.x + b.z * map.y) + nrm.z * map.z});
+    }
+    // Ambient SH (:40-57, :248-249).
+    const float L = 0.025f;
+    float shv = (((((((L + 0.0f * nrm.x) + L * nrm.y) + 0.0f * nrm.z) + 0.0f * (nrm.x * nrm.z)) + 0.0f * (nrm.z * nrm.y)) +
+                  0.0f * (nrm.y * nrm.x)) +
+                 0.0f * ((3.0f * nrm.z) * nrm.z - 1.0f)) +
+                0.0f * (nrm.x * nrm.x - nrm.y * nrm.y);
+    F3 ambient = mul(diffuse_color(base, metallic), shv);
+    float sun_scale = kPi * shadow_sum;
+    F3 sun = mul(mul(brdf(view_vector, f3(u.sun_dir), nrm, roughness, metallic, base,
+                          (mi.flags & MESH_INFO_FLAGS_ALPHA_CLIP) != 0),
+                     sun_scale),
+                 f3(u.sun_intensity));
+    return add(sun, ambient);
+}
+
+// display_transform (src/shaders/display_transform.comp:5-33): the
+// tony-mc-mapface LUT is absent, so the LUT's own pre-encoding x/(x+1) is
+// used as the tone curve, followed by the sRGB OETF and BGRA8 packing.
+float srgb_oetf(float x) { return x <= 0.003130f ? x * 12.92f : 1.055f * std::pow(x, 1.0f / 2.4f) - 0.055f; }
+uint32_t tonemap_pixel(F3 c) {
+    auto enc = [](float x) {
+        x = std::max(x, 0.0f);
+        float t = srgb_oetf(x / (x + 1.0f));
+        t = clampf(t, 0.0f, 1.0f);
+        return uint32_t(t * 255.0f + 0.5f);
+    };
+    return enc(c.z) | (enc(c.y) << 8) | (enc(c.x) << 16) | (255u << 24);
+}
+
+bool valid(const lh_scene* sc, const lh_view* v) {
+    return sc && v && v->uniforms && v->misc && v->prefix_sums && v->draws && v->counters && v->width > 0 &&
+           v->height > 0 && v->width <= LH_MAX_VIEW_SIDE && v->height <= LH_MAX_VIEW_SIDE;
+}
+
+}  // namespace
+
+extern "C" {
+
+lh_status orc_view_reset(const lh_scene* sc, const lh_view* v) {
+    if (!valid(sc, v)) return LH_ERR_INVALID;
+    std::memset(v->counters, 0, LH_NUM_COUNTERS * 4);
+    for (uint32_t p = 0; p < LH_NUM_PASSES; p++) sector(*v, p)[0] = 0;
+    v->misc->min_depth = 0xFFFFFFFFu;  // reset_buffers_a (reset_buffers.glsl:9-13)
+    v->misc->max_depth = 0;
+    if (v->visbuffer) std::fill(v->visbuffer, v->visbuffer + size_t(v->width) * v->height, LH_VISBUFFER_CLEAR);
+    if (v->shadow_maps)
+        std::fill(v->shadow_maps, v->shadow_maps + size_t(LH_NUM_CASCADES) * v->shadow_size * v->shadow_size,
+                  LH_SHADOW_CLEAR);
+    return LH_OK;
+}
+
+// cull_instances (src/shaders/cull_instances.comp:7-27).
+lh_status orc_cull_instances(const lh_scene* sc, const lh_view* v) {
+    if (!valid(sc, v)) return LH_ERR_INVALID;
+    const Uniforms& u = *v->uniforms;
+    uint32_t n = std::min(u.num_instances, sc->num_instances);
+    for (uint32_t i = 0; i < n; i++) {
+        const Instance& in = sc->instances[i];
+        const MeshInfo& mi = mesh_info_of(in);
+        if (cull_sphere(in, &mi.bounding_sphere.x, u)) continue;
+        prefix_append(*v, 0, i, mi.num_meshlets);
+    }
+    return LH_OK;
+}
+
+// write_draw_calls / write_draw_calls_shadows (src/shaders/write_draw_calls.comp:7-117)
+// with get_meshlet_reference (src/shaders/common/util.glsl:42-59).
+lh_status orc_cull_meshlets(const lh_scene* sc, const lh_view* v, uint32_t pass) {
+    if (!valid(sc, v) || pass >= LH_NUM_PASSES) return LH_ERR_INVALID;
+    const Uniforms& u = *v->uniforms;
+    uint32_t total = uint32_t(sector(*v, pass)[0]);
+    MeshletReference* out = v->draws + size_t(pass) * v->draw_capacity;
+    for (uint32_t id = 0; id < total; id++) {
+        PrefixSumValue e;
+        if (!prefix_search(*v, pass, id, e)) continue;
+        const Instance& in = sc->instances[e.index];
+        const MeshInfo& mi = mesh_info_of(in);
+        uint32_t local = id - (e.sum - mi.num_meshlets);
+        const Meshlet& ml = meshlet_of(mi, local);
+        bool alpha = (mi.flags & MESH_INFO_FLAGS_ALPHA_CLIP) != 0;
+        if (pass == 0) {
+            if (cull_sphere(in, &ml.bounding_sphere.x, u)) continue;
+            if (!alpha && cull_cone(in, ml, u, false)) continue;
+        } else {
+            if (cull_sphere_shadow(in, &ml.bounding_sphere.x, *v->misc, pass - 1)) continue;
+            if (!alpha && cull_cone(in, ml, u, true)) continue;
+        }
+        uint32_t slot = v->counters[LH_CNT_DRAWS + pass]++;
+        if (slot >= v->draw_capacity) {
+            v->counters[LH_CNT_OVERFLOW] |= LH_OVERFLOW_DRAWS;
+            continue;
+        }
+        out[slot].instance_index = e.index;
+        out[slot].meshlet_index = uint16_t(local);
+    }
+    return LH_OK;
+}
+
+lh_status orc_raster_view(const lh_scene* sc, const lh_view* v) {
+    if (!valid(sc, v) || !v->visbuffer) return LH_ERR_INVALID;
+    raster_pass(*sc, *v, 0);
+    return LH_OK;
+}
+
+lh_status orc_raster_shadow(const lh_scene* sc, const lh_view* v, uint32_t cascade) {
+    if (!valid(sc, v) || !v->shadow_maps || cascade >= LH_NUM_CASCADES) return LH_ERR_INVALID;
+    raster_pass(*sc, *v, 1 + cascade);
+    return LH_OK;
+}
+
+// read_depth (src/shaders/compute/read_depth.glsl:3-83): min over non-zero
+// depth bits, max over all; depth bits are ~(key >> 32).
+lh_status orc_depth_bounds(const lh_scene* sc, const lh_view* v) {
+    if (!valid(sc, v) || !v->visbuffer) return LH_ERR_INVALID;
+    uint32_t mn = v->misc->min_depth, mx = v->misc->max_depth;
+    for (size_t p = 0; p < size_t(v->width) * v->height; p++) {
+        uint32_t d = ~uint32_t(v->visbuffer[p] >> 32);
+        if (d != 0) mn = std::min(mn, d);
+        mx = std::max(mx, d);
+    }
+    v->misc->min_depth = mn;
+    v->misc->max_depth = mx;
+    if (mn == 0xFFFFFFFFu) v->counters[LH_CNT_FLAGS] |= LH_FLAG_EMPTY_VIEW;
+    return LH_OK;
+}
+
+// generate_shadow_matrices (src/shaders/compute/generate_shadow_matrices.glsl:15-110)
+// with lookAt / OrthographicProjection (src/shaders/common/matrices.glsl:9-40).
+lh_status orc_fit_cascades(const lh_scene* sc, const lh_view* v) {
+    if (!valid(sc, v)) return LH_ERR_INVALID;
+    const Uniforms& u = *v->uniforms;
+    MiscStorage& m = *v->misc;
+    float min_depth = bitsf(m.min_depth), max_depth = bitsf(m.max_depth);
+    float nearp = near_plane(&u);
+    float min_distance = nearp / max_depth, max_distance = nearp / min_depth;
+    float splits[4] = {max_depth, nearp / mixf(min_distance, max_distance, split_pow(0.5f, u.cascade_split_pow)),
+                       nearp / mixf(min_distance, max_distance, split_pow(0.75f, u.cascade_split_pow)), min_depth};
+    for (uint32_t c = 0; c < 4; c++) {
+        float last = c > 0 ? splits[c - 1] : max_depth, split = splits[c];
+        F3 corners[8] = {{-1, 1, last}, {1, 1, last}, {1, -1, last}, {-1, -1, last},
+                         {-1, 1, split}, {1, 1, split}, {1, -1, split}, {-1, -1, split}};
+        for (auto& k : corners) {
+            F4 q = mat4_mul_vec4(u.inv_perspective_view, F4{k.x, k.y, k.z, 1.0f});
+            k = F3{q.x / q.w, q.y / q.w, q.z / q.w};
+        }
+        F3 centre{0, 0, 0};
+        for (auto& k : corners) centre = add(centre, k);
+        centre = F3{centre.x / 8.0f, centre.y / 8.0f, centre.z / 8.0f};
+        float radius = 0.0f;
+        for (auto& k : corners) radius = std::max(radius, length3(sub(k, centre)));
+        radius = std::ceil(radius * 16.0f) / 16.0f;
+        float dist = u.shadow_cam_distance;
+        F3 eye = add(mul(f3(u.sun_dir), dist), centre);
+        // lookAt(eye, centre, +Y)
+        F3 f = normalize3(sub(centre, eye));
+        F3 r = normalize3(cross3(f, F3{0, 1, 0}));
+        F3 up = cross3(r, f);
+        LhMat4 view;
+        view.c[0] = LhVec4{r.x, up.x, -f.x, 0};
+        view.c[1] = LhVec4{r.y, up.y, -f.y, 0};
+        view.c[2] = LhVec4{r.z, up.z, -f.z, 0};
+        view.c[3] = LhVec4{-dot3(r, eye), -dot3(up, eye), dot3(f, eye), 1};
+        // OrthographicProjection(-r, r, -r, r, 0, 2 dist)
+        float left = -radius, right = radius, bottom = -radius, top = radius, zn = 0.0f, zf = dist * 2.0f;
+        float rml = right - left, rpl = right + left, tmb = top - bottom, tpb = top + bottom, fmn = zf - zn;
+        LhMat4 proj;
+        proj.c[0] = LhVec4{2.0f / rml, 0, 0, 0};
+        proj.c[1] = LhVec4{0, -2.0f / tmb, 0, 0};
+        proj.c[2] = LhVec4{0, 0, -1.0f / fmn, 0};
+        proj.c[3] = LhVec4{-(rpl / rml), -(tpb / tmb), -(zn / fmn), 1};
+        LhMat4 mat = mat4_mul(proj, view);
+        LhMat4 bias;
+        bias.c[0] = LhVec4{0.5f, 0, 0, 0};
+        bias.c[1] = LhVec4{0, 0.5f, 0, 0};
+        bias.c[2] = LhVec4{0, 0, 1, 0};
+        bias.c[3] = LhVec4{0.5f, 0.5f, 0, 1};
+        m.shadow_matrices[c] = mat;
+        m.uv_space_shadow_matrices[c] = mat4_mul(bias, mat);
+        m.shadow_view_matrices[c] = view;
+        m.shadow_sphere_radii[c] = radius;
+    }
+    return LH_OK;
+}
+
+// cull_instances_shadows (src/shaders/cull_instances.comp:31-56).
+lh_status orc_cull_instances_shadow(const lh_scene* sc, const lh_view* v) {
+    if (!valid(sc, v)) return LH_ERR_INVALID;
+    uint32_t n = std::min(v->uniforms->num_instances, sc->num_instances);
+    for (uint32_t c = 0; c < LH_NUM_CASCADES; c++)
+        for (uint32_t i = 0; i < n; i++) {
+            const Instance& in = sc->instances[i];
+            const MeshInfo& mi = mesh_info_of(in);
+            if (cull_sphere_shadow(in, &mi.bounding_sphere.x, *v->misc, c)) continue;
+            prefix_append(*v, 1 + c, i, mi.num_meshlets);
+        }
+    return LH_OK;
+}
+
+lh_status orc_resolve(const lh_scene* sc, const lh_view* v) {
+    if (!valid(sc, v) || !v->visbuffer || !v->radiance) return LH_ERR_INVALID;
+    for (uint32_t y = 0; y < v->height; y++)
+        for (uint32_t x = 0; x < v->width; x++) {
+            F3 c = resolve_pixel(*sc, *v, x, y);
+            float* o = v->radiance + 4 * (size_t(y) * v->width + x);
+            o[0] = c.x;
+            o[1] = c.y;
+            o[2] = c.z;
+            o[3] = 1.0f;
+            if (v->ldr) v->ldr[size_t(y) * v->width + x] = tonemap_pixel(c);
+        }
+    return LH_OK;
+}
+
+lh_status orc_tonemap(const lh_view* v) {
+    if (!v || !v->radiance || !v->ldr) return LH_ERR_INVALID;
+    for (size_t p = 0; p < size_t(v->width) * v->height; p++) {
+        const float* c = v->radiance + 4 * p;
+        v->ldr[p] = tonemap_pixel(F3{c[0], c[1], c[2]});
+    }
+    return LH_OK;
+}
+
+// render() pass order (src/rendering.cpp:79-502).
+lh_status orc_frame(const lh_scene* sc, const lh_view* v) {
+    lh_status s;
+    if ((s = orc_view_reset(sc, v))) return s;
+    if ((s = orc_cull_instances(sc, v))) return s;
+    if ((s = orc_cull_meshlets(sc, v, 0))) return s;
+    if ((s = orc_raster_view(sc, v))) return s;
+    if ((s = orc_depth_bounds(sc, v))) return s;
+    if ((s = orc_fit_cascades(sc, v))) return s;
+    if ((s = orc_cull_instances_shadow(sc, v))) return s;
+    for (uint32_t c = 0; c < LH_NUM_CASCADES; c++) {
+        if ((s = orc_cull_meshlets(sc, v, 1 + c))) return s;
+        if ((s = orc_raster_shadow(sc, v, c))) return s;
+    }
+    if ((s = orc_resolve(sc, v))) return s;
+    return v->counters[LH_CNT_OVERFLOW] ? LH_ERR_CAPACITY : LH_OK;
+}
+
+uint64_t orc_raster_clip_triangle(const float* clip, uint32_t width, uint32_t height, uint32_t kind, uint32_t id,
+                                  uint64_t* vis64, uint32_t* depth32) {
+    CV c[3];
+    for (int i = 0; i < 3; i++) c[i] = CV{clip[4 * i], clip[4 * i + 1], clip[4 * i + 2], clip[4 * i + 3]};
+    RasterTarget t{width, height, kind, vis64, depth32};
+    return raster_clip_triangle(c, t, id);
+}
+
+int orc_cull_sphere(const Instance* in, const float* s, const Uniforms* u) { return cull_sphere(*in, s, *u) ? 1 : 0; }
+int orc_cull_sphere_shadow(const Instance* in, const float* s, const MiscStorage* m, uint32_t c) {
+    return cull_sphere_shadow(*in, s, *m, c) ? 1 : 0;
+}
+int orc_cull_cone(const Instance* in, const Meshlet* ml, const Uniforms* u, int ortho) {
+    return cull_cone(*in, *ml, *u, ortho != 0) ? 1 : 0;
+}
+void orc_calc_full_bary(const float* a, const float* b, const float* c, const float* ndc, const float* win, float* o) {
+    BaryDeriv d = calc_full_bary(F4{a[0], a[1], a[2], a[3]}, F4{b[0], b[1], b[2], b[3]}, F4{c[0], c[1], c[2], c[3]},
+                                 F2{ndc[0], ndc[1]}, F2{win[0], win[1]});
+    float r[9] = {d.lambda.x, d.lambda.y, d.lambda.z, d.ddx.x, d.ddx.y, d.ddx.z, d.ddy.x, d.ddy.y, d.ddy.z};
+    std::memcpy(o, r, sizeof(r));
+}
+void orc_brdf(const float* v, const float* l, const float* n, float r, float m, const float* base, int thin, float* o) {
+    F3 x = brdf(F3{v[0], v[1], v[2]}, F3{l[0], l[1], l[2]}, F3{n[0], n[1], n[2]}, r, m, F3{base[0], base[1], base[2]},
+                thin != 0);
+    o[0] = x.x;
+    o[1] = x.y;
+    o[2] = x.z;
+}
+float orc_srgb_oetf(float x) { return srgb_oetf(x); }
+
+}  // extern "C"

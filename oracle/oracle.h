@@ -1,0 +1,69 @@
+/*
+ * Scalar C++ oracle of the GPU-driven frame.  TEST INFRASTRUCTURE ONLY:
+ * loaded by tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg
+ * as the checker; never part of the product path.
+ *
+ * Every function restates the reference's algorithm for one pass, citing the
+ * file:line it follows, in plain scalar C++ compiled without FMA contraction
+ * and with IEEE division / square root, so that culling decisions, prefix
+ * sums and visibility-buffer keys are reproducible bit for bit by the HIP
+ * kernels.  The entry points take the same lh_scene / lh_view structs as
+ * include/lh/lh.h, but every address in them (and inside Instance / MeshInfo
+ * / Uniforms) is a host address.
+ *
+ * The software raster rules have no reference code (the reference uses the
+ * Vulkan fixed-function rasterizer); they are this project's restatement of
+ * the Vulkan rasterization rules: 8-bit subpixel snapping, pixel-centre
+ * sampling, top-left fill rule, CCW front faces in framebuffer coordinates,
+ * clipping to 0 <= z <= w and to a guard band |x|,|y| <= 4w, screen-linear
+ * depth clamped to [0, 1].
+ */
+#ifndef LH_ORACLE_H
+#define LH_ORACLE_H
+
+#include "lh/lh.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+lh_status orc_view_reset(const lh_scene* scene, const lh_view* view);
+lh_status orc_cull_instances(const lh_scene* scene, const lh_view* view);
+lh_status orc_cull_meshlets(const lh_scene* scene, const lh_view* view, uint32_t pass);
+lh_status orc_raster_view(const lh_scene* scene, const lh_view* view);
+lh_status orc_depth_bounds(const lh_scene* scene, const lh_view* view);
+lh_status orc_fit_cascades(const lh_scene* scene, const lh_view* view);
+lh_status orc_cull_instances_shadow(const lh_scene* scene, const lh_view* view);
+lh_status orc_raster_shadow(const lh_scene* scene, const lh_view* view, uint32_t cascade);
+lh_status orc_resolve(const lh_scene* scene, const lh_view* view);
+lh_status orc_tonemap(const lh_view* view);
+lh_status orc_frame(const lh_scene* scene, const lh_view* view);
+
+/* ---- building blocks exposed for known-answer tests ---- */
+
+/* Rasterizes one clip-space triangle (clip[v][x,y,z,w]).  kind 0: main view
+ * (back faces culled, 64-bit keys into vis64 with id = meshlet << 7 | tri);
+ * kind 1: shadow (front faces culled, float bits into depth32).
+ * Returns the number of fragments that passed coverage. */
+uint64_t orc_raster_clip_triangle(const float* clip, uint32_t width, uint32_t height, uint32_t kind,
+                                  uint32_t id, uint64_t* vis64, uint32_t* depth32);
+
+/* Sphere tests of src/shaders/common/culling.glsl:1-46 and :82-135 (1 = culled). */
+int orc_cull_sphere(const Instance* instance, const float* sphere, const Uniforms* u);
+int orc_cull_sphere_shadow(const Instance* instance, const float* sphere, const MiscStorage* misc, uint32_t cascade);
+/* Cone tests of src/shaders/common/culling.glsl:137-151 (1 = culled). */
+int orc_cull_cone(const Instance* instance, const Meshlet* meshlet, const Uniforms* u, int orthographic);
+
+/* CalcFullBary (src/shaders/common/vbuffer.glsl:11-63): out = lambda[3], ddx[3], ddy[3]. */
+void orc_calc_full_bary(const float* pt0, const float* pt1, const float* pt2, const float* pixel_ndc,
+                        const float* win_size, float* out9);
+/* BRDF (src/shaders/common/pbr.glsl:44-96): v, l, n, roughness, metallic, base colour -> rgb. */
+void orc_brdf(const float* v, const float* l, const float* n, float roughness, float metallic,
+              const float* base_color, int thin_walled, float* out3);
+/* sRGB OETF (src/shaders/display_transform.comp:19-25). */
+float orc_srgb_oetf(float x);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* LH_ORACLE_H */
