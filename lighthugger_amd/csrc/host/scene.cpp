@@ -945,8 +945,11 @@ int32_t lh_preset_camera(uint32_t preset, lh_camera* c) {
             return 0;
         case LH_PRESET_LARGE:
         case LH_PRESET_STRESS:
-            c->position = LhVec3{0, 40, 330};
-            c->target = LhVec3{0, 0, 60};
+            // Inside the clustered field looking along +X: ~10.6M triangles of
+            // instances survive the main-view instance cull for the 2,000-instance
+            // scene (~10x that for 20,000 instances).
+            c->position = LhVec3{0, 15, 0};
+            c->target = LhVec3{100, 5, 0};
             c->near_plane = 0.1f;
             c->width = preset == LH_PRESET_LARGE ? 1920 : 3840;
             c->height = preset == LH_PRESET_LARGE ? 1080 : 2160;

@@ -814,7 +814,6 @@ lh_status orc_depth_bounds(const lh_scene* sc, const lh_view* v) {
     }
     v->misc->min_depth = mn;
     v->misc->max_depth = mx;
-    if (mn == 0xFFFFFFFFu) v->counters[LH_CNT_FLAGS] |= LH_FLAG_EMPTY_VIEW;
     return LH_OK;
 }
 
@@ -824,6 +823,7 @@ lh_status orc_fit_cascades(const lh_scene* sc, const lh_view* v) {
     if (!valid(sc, v)) return LH_ERR_INVALID;
     const Uniforms& u = *v->uniforms;
     MiscStorage& m = *v->misc;
+    if (m.min_depth == 0xFFFFFFFFu) v->counters[LH_CNT_FLAGS] |= LH_FLAG_EMPTY_VIEW;
     float min_depth = bitsf(m.min_depth), max_depth = bitsf(m.max_depth);
     float nearp = near_plane(&u);
     float min_distance = nearp / max_depth, max_distance = nearp / min_depth;

@@ -1,0 +1,150 @@
+"""Scene producer invariants (C++ host, include/lh/lh_scene.h).
+
+Meshlets: <= 64 vertices, <= 124 triangles, u8 micro indices in range,
+triangle blocks 4-byte aligned (src/resources/meshlets.cpp:62-64), every source
+triangle exactly once.  Bounds: spheres contain their vertices; the stable
+meshlet table inverts the instance-order scan; the normal matrix is
+transpose(inverse(M)); uniforms follow src/projection.h:3-20.
+meshoptimizer's exact meshlet grouping is not reproduced (parity unpinned:
+the library is absent); the kernels and the oracle consume the same meshlets.
+"""
+import numpy as np
+import pytest
+
+from lighthugger_amd import abi
+from lighthugger_amd.scene import HostScene, make_camera, make_uniforms, preset_camera
+
+from .util import cube_mesh, grid_patch, single_scene, translate
+
+
+@pytest.fixture(scope="module")
+def small():
+    return HostScene.preset(abi.PRESET_SMALL)
+
+
+def _mesh_arrays(scene, mi, nverts):
+    ml = scene.meshlets(mi)
+    last = ml[-1]
+    remap = scene.remap(mi, int(last["index_offset"]) + int(last["index_count"]))
+    micro = scene.micro(mi, int(last["triangle_offset"]) + 3 * int(last["triangle_count"]))
+    pos = scene.positions(mi, nverts)
+    return ml, remap, micro, pos
+
+
+def test_small_preset_shape(small):
+    assert small.num_instances == 64
+    assert small.num_meshes == 1
+    assert small.unique_triangles == 8192
+    assert small.total_triangles == 64 * 8192
+    assert len(small.mesh_infos) == 4  # one sphere x 4 materials
+
+
+def _check_meshlets(scene, mi, nverts, src_idx=None):
+    ml, remap, micro, pos = _mesh_arrays(scene, mi, nverts)
+    assert (ml["index_count"] <= 64).all() and (ml["triangle_count"] <= 124).all()
+    assert (ml["triangle_count"] > 0).all()
+    assert (ml["triangle_offset"] % 4 == 0).all()
+    tris = []
+    for m in ml:
+        t0, nt, i0, nv = int(m["triangle_offset"]), int(m["triangle_count"]), int(m["index_offset"]), int(m["index_count"])
+        c = micro[t0: t0 + 3 * nt].reshape(nt, 3)
+        assert c.max() < nv
+        v = remap[i0 + c.astype(np.int64)]
+        tris.append(v)
+        p = pos[remap[i0: i0 + nv]]
+        cs, r = m["bounding_sphere"][:3], m["bounding_sphere"][3]
+        d = np.linalg.norm(p[np.unique(c)] - cs, axis=1)
+        assert (d <= r * (1 + 1e-5) + 1e-6).all(), "meshlet sphere must contain its vertices"
+    tris = np.concatenate(tris)
+    if src_idx is not None:
+        key = lambda t: np.sort(t, axis=1)  # noqa: E731
+        a = key(tris)
+        b = key(src_idx)
+        a = a[np.lexsort(a.T[::-1])]
+        b = b[np.lexsort(b.T[::-1])]
+        assert np.array_equal(a, b), "meshlets must hold every source triangle exactly once"
+    sph = scene.mesh_infos[mi]["bounding_sphere"]
+    d = np.linalg.norm(pos - sph[:3], axis=1)
+    assert (d <= sph[3] * (1 + 1e-5) + 1e-6).all()
+    return ml
+
+
+def test_small_meshlets(small):
+    ml = _check_meshlets(small, 0, 4098)
+    assert ml["triangle_count"].mean() > 80
+
+
+def test_grid_and_cube_meshlets_cover_every_triangle():
+    for mesh in (grid_patch(20, 17), cube_mesh()):
+        s = single_scene(mesh, [np.eye(4, dtype=np.float32)])
+        _check_meshlets(s, 0, len(mesh[0]), mesh[3])
+
+
+def test_mesh_at_exactly_64_vertices_and_124_triangles():
+    # A fan of 62 rim vertices + centre... build a strip with 64 vertices and
+    # <= 124 triangles: 2 x 32 vertex strip has 62 triangles; two stacked rows
+    # give 3 x 21 = 63 vertices and 80 triangles.  Use a 8 x 8 grid: 64 verts, 98 tris.
+    mesh = grid_patch(7, 7)
+    assert len(mesh[0]) == 64 and len(mesh[3]) == 98
+    s = single_scene(mesh, [np.eye(4, dtype=np.float32)])
+    ml = _check_meshlets(s, 0, 64, mesh[3])
+    assert len(ml) == 1 and ml[0]["index_count"] == 64 and ml[0]["triangle_count"] == 98
+
+
+def test_stable_meshlet_table(small):
+    base = small.instance_meshlet_base
+    nm = small.mesh_infos["num_meshlets"][small.instances["mesh_info_address"].astype(np.int64)].astype(np.int64)
+    assert np.array_equal(base.astype(np.int64), np.concatenate([[0], np.cumsum(nm)[:-1]]))
+    t = small.meshlet_table
+    assert len(t) == nm.sum()
+    for i in (0, 17, 63):
+        seg = t[base[i]: base[i] + nm[i]]
+        assert (seg["instance_index"] == i).all()
+        assert np.array_equal(seg["meshlet_index"], np.arange(nm[i]))
+
+
+def test_normal_matrix_is_inverse_transpose(small):
+    for k in range(0, 64, 9):
+        m = np.asarray(small.instances["transform"][k], np.float64).T  # row-major
+        n = np.asarray(small.instances["normal_transform"][k], np.float64).T
+        np.testing.assert_allclose(n, np.linalg.inv(m[:3, :3]).T, rtol=1e-5, atol=1e-6)
+
+
+def test_uniforms_follow_infinite_reverse_z():
+    cam = make_camera((1, 2, 3), (0, 0, 0), 1280, 720, fov_deg=60, near=0.1)
+    u = make_uniforms(cam, 7)[0]
+    P = np.asarray(u["perspective"], np.float64).T
+    f = 1 / np.tan(np.radians(30))
+    np.testing.assert_allclose(P, [[f / (1280 / 720), 0, 0, 0], [0, -f, 0, 0], [0, 0, 0, 0.1], [0, 0, -1, 0]],
+                               rtol=1e-6, atol=1e-7)
+    V = np.asarray(u["view"], np.float64).T
+    np.testing.assert_allclose(V @ [1, 2, 3, 1], [0, 0, 0, 1], atol=1e-5)
+    # reverse-Z: a point at distance d has depth near / d
+    fwd = -np.array([1, 2, 3]) / np.linalg.norm([1, 2, 3])
+    q = np.asarray(u["combined_perspective_view"], np.float64).T @ np.append(np.array([1, 2, 3]) + 10 * fwd, 1)
+    assert abs(q[2] / q[3] - 0.01) < 1e-6
+    assert u["num_instances"] == 7 and tuple(u["window_size"]) == (1280, 720)
+    inv = np.asarray(u["inv_perspective_view"], np.float64).T
+    np.testing.assert_allclose(inv @ np.asarray(u["combined_perspective_view"], np.float64).T, np.eye(4), atol=1e-4)
+
+
+def test_large_preset_statistics():
+    s = HostScene.preset(abi.PRESET_LARGE, texture_size=16, tri_scale_pct=5)
+    assert s.num_instances == 2000 and s.num_meshes == 50
+    assert len(s.textures) == 64
+    assert (s.textures["levels"] == 5).all()
+    tri = s.instance_triangles()
+    assert tri.sum() == s.total_triangles
+
+
+def test_preset_cameras():
+    for p, (w, h) in ((abi.PRESET_SMALL, (1280, 720)), (abi.PRESET_LARGE, (1920, 1080)), (abi.PRESET_STRESS, (3840, 2160))):
+        c = preset_camera(p)
+        assert (c.width, c.height) == (w, h)
+        assert abs(c.near_plane - 0.1) < 1e-7
+
+
+def test_invalid_inputs_rejected():
+    pos = np.zeros((3, 3), np.float32)
+    with pytest.raises(ValueError):
+        HostScene.build([(pos, None, None, np.array([[0, 1, 5]], np.uint32))], [], [])
