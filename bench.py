@@ -1,0 +1,318 @@
+"""Benchmark of the GPU-driven frame on one MI355X.
+
+One step = one whole frame of the large workload through the C ABI
+(lh_frame): reset, instance cull + 64-bit-atomic prefix sum, meshlet cull,
+visibility-buffer raster at 1920x1080, depth bounds, cascade fitting, shadow
+instance/meshlet cull and raster of 4 cascades at 2048^2, shading resolve with
+fused tonemap.  Inputs are resident in HBM before timing starts.
+
+Prints one JSON line (rank 0, N = 1).  `value` is ms per frame (lower is
+better); Gtri/s uses the visible-candidate triangles (triangles of the
+instances surviving the main-view instance cull).  `roofline` is for the
+kernel with the largest device time per frame, timed live with HIP events
+recorded on the frame's stream; `cpu_baseline` times the scalar oracle on
+host cores over a bounded sample of the same frames.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+METRIC = "ms/frame & Gtri/s (cull+raster+shade) 10M tris 1080p; batched views/s vs plain"
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--workload", choices=["large", "small", "stress"], default="large")
+    ap.add_argument("--cpu-frames", type=int, default=4, help="oracle frames timed for the CPU baseline (0 = skip)")
+    ap.add_argument("--batch-steps", type=int, default=10, help="batched-views timing iterations (0 = skip)")
+    ap.add_argument("--profile-only", action="store_true", help="run a few frames for an external profiler")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="run 256 MiB device-to-device copies (HBM-counter calibration for rocprofv3 --pmc)")
+    return ap.parse_args()
+
+
+def copy_probe(steps: int) -> None:
+    import torch
+
+    n = 256 << 20
+    a = torch.empty(n, dtype=torch.uint8, device="cuda")
+    b = torch.empty_like(a)
+    a.fill_(1)
+    for _ in range(max(1, steps)):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(f"# copy probe: {steps} copies of {n} bytes", file=sys.stderr)
+
+
+def meshlet_sizes(scene):
+    """(index_count, triangle_count) for every (mesh_info, meshlet), with per-mesh-info offsets."""
+    offs, nv, nt = [0], [], []
+    for i in range(len(scene.mesh_infos)):
+        ml = scene.meshlets(i)
+        nv.append(ml["index_count"].astype(np.int64))
+        nt.append(ml["triangle_count"].astype(np.int64))
+        offs.append(offs[-1] + len(ml))
+    return np.array(offs[:-1]), np.concatenate(nv), np.concatenate(nt)
+
+
+def raster_algorithmic_bytes(scene, draws, covered_pixels, sizes):
+    """Compulsory HBM bytes of the per-meshlet raster of one pass: per surviving
+    meshlet the draw record (8), meshlet header (56), stable-id base (4), remap
+    (4 nv), FP32 positions (12 nv) and micro indices (3 nt, 4-byte padded); per
+    distinct instance its Instance (112) and MeshInfo (104) records; one 8-byte
+    key per covered pixel."""
+    offs, nv_all, nt_all = sizes
+    inst = draws["instance_index"].astype(np.int64)
+    mi = scene.instances["mesh_info_address"].astype(np.int64)[inst]
+    g = offs[mi] + draws["meshlet_index"].astype(np.int64)
+    nv, nt = nv_all[g], nt_all[g]
+    per_meshlet = 8 + 56 + 4 + 16 * nv + ((3 * nt + 3) & ~3)
+    return int(per_meshlet.sum() + 216 * len(np.unique(inst)) + 8 * covered_pixels)
+
+
+def batch_bench(scene, steps: int, n_cameras: int = 64):
+    """Batched views: n_cameras seeded square cameras (side log-uniform in
+    [256, 2048], multiples of 8), each with its own 4 cascades of the same side
+    (5 views per camera).  Times the per-view loop and the batched launch, each
+    with direct launches and as a replayed hipGraph, and checks on the device
+    that every view's outputs are bit-identical between loop and batch."""
+    import torch
+
+    from lighthugger_amd.lib import BATCH_BATCHED, BATCH_LOOP, BatchRenderer
+    from lighthugger_amd.scene import batch_cameras
+
+    from lighthugger_amd import abi
+
+    cams = batch_cameras(4, n_cameras, 256, 2048)
+    br = BatchRenderer(scene, cams, big_capacity=1 << 20)
+    n_views = n_cameras * 5
+
+    def overflow() -> int:
+        return int(np.bitwise_or.reduce([int(v.counters()[abi.CNT_OVERFLOW]) for v in br.views]))
+
+    def outputs():
+        out = []
+        for v in br.views:
+            t = v.handle[0]
+            for name in ("visbuffer", "shadow", "radiance", "ldr"):
+                o, n = v.offsets[name], v.sizes[name]
+                out.append(t[o: o + n])
+        return out
+
+    rates = {}
+    for name, mode, graph in (("loop", BATCH_LOOP, False), ("loop_graph", BATCH_LOOP, True),
+                              ("batched", BATCH_BATCHED, False), ("batched_graph", BATCH_BATCHED, True)):
+        br.run(mode, graph)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            br.run(mode, graph)
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t0) / steps
+        rates[name] = {"ms_per_batch": round(dt * 1000, 3), "views_per_s": round(n_views / dt, 1)}
+        ovf = overflow()
+        if ovf:
+            raise RuntimeError(f"batched views: capacity overflow {ovf:#x} in mode {name}")
+        if name == "loop":
+            ref = [o.clone() for o in outputs()]
+        elif name in ("batched", "batched_graph"):
+            same = all(torch.equal(a, b) for a, b in zip(ref, outputs()))
+            rates[name]["bit_identical_to_loop"] = bool(same)
+    br.close()
+    return {
+        "cameras": n_cameras, "views": n_views, "sides": [int(c.width) for c in cams],
+        "pixels_per_batch": int(sum(c.width * c.height * 5 for c in cams)),
+        "modes": rates,
+        "speedup_vs_loop": round(rates["batched_graph"]["views_per_s"] / rates["loop"]["views_per_s"], 3),
+        "speedup_vs_loop_graph": round(rates["batched_graph"]["views_per_s"] / rates["loop_graph"]["views_per_s"], 3),
+    }
+
+
+def _tuning_of(cls: int) -> dict:
+    import ctypes as C
+
+    from lighthugger_amd.lib import Tuning, lib
+
+    t = Tuning()
+    lib().lh_get_tuning_class(cls, C.byref(t))
+    return t.as_dict()
+
+
+def cpu_baseline(scene, cfg, uniforms, frames):
+    import oracle
+    from oracle.compare import oracle_view
+
+    sd, vd = oracle_view(scene, cfg, uniforms)
+    L = oracle.lib()
+    L.orc_frame(sd.address, vd.address)  # warm-up (page faults, caches)
+    t0 = time.perf_counter()
+    for _ in range(frames):
+        L.orc_frame(sd.address, vd.address)
+    dt = (time.perf_counter() - t0) / frames
+    cpu = ""
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                cpu = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return {"value": dt * 1000.0, "unit": "ms/frame", "cores": 1, "kind": "port",
+            "sample": f"{frames} whole frames of the same workload (after 1 warm-up) through the scalar "
+                      f"oracle, single thread on {cpu or 'the host'} (nproc={os.cpu_count()})"}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path (the frame would shard "
+              "sort-last over RCCL; see the design notes). Run with --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+    if args.copy_probe:
+        copy_probe(args.steps)
+        return
+    import torch
+
+    from lighthugger_amd import abi
+    from lighthugger_amd.frame import ViewConfig
+    from lighthugger_amd.lib import Renderer
+    from lighthugger_amd.scene import HostScene, make_uniforms, preset_camera
+
+    preset = {"large": abi.PRESET_LARGE, "small": abi.PRESET_SMALL, "stress": abi.PRESET_STRESS}[args.workload]
+    shadow = {"large": 2048, "small": 1024, "stress": 4096}[args.workload]
+    t0 = time.time()
+    scene = HostScene.preset(preset)
+    cam = preset_camera(preset)
+    cfg = ViewConfig(cam.width, cam.height, shadow, big_capacity=1 << 21, clip_capacity=1 << 18)
+    r = Renderer(scene, cfg)
+    r.set_camera(cam)
+    print(f"# scene built in {time.time() - t0:.1f}s: {scene.num_instances} instances, "
+          f"{scene.total_triangles} triangles", file=sys.stderr, flush=True)
+
+    for _ in range(max(1, args.warmup)):
+        r.frame()
+    torch.cuda.synchronize()
+    st = r.status()
+    if st[abi.CNT_OVERFLOW]:
+        raise RuntimeError(f"capacity overflow during warm-up: {st[abi.CNT_OVERFLOW]:#x}")
+    queue_counts = {"big": [int(x) for x in st[abi.CNT_BIG: abi.CNT_BIG + 2]],
+                    "clip": [int(x) for x in st[abi.CNT_CLIP: abi.CNT_CLIP + 2]]}
+    if args.profile_only:
+        for _ in range(args.steps):
+            r.frame()
+        torch.cuda.synchronize()
+        print("# profile-only run done", file=sys.stderr)
+        return
+
+    # ---- timed region: exactly `steps` frames ----
+    torch.cuda.synchronize()
+    t_start = time.perf_counter()
+    for _ in range(args.steps):
+        r.frame()
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t_start) * 1000.0 / args.steps
+
+    # ---- per-kernel device times over the same number of frames ----
+    acc: dict[str, list[float]] = {}
+    for _ in range(args.steps):
+        for name, k_ms in r.frame_profiled():
+            acc.setdefault(name, []).append(k_ms)
+    # kernels launched several times per frame (per cascade) are summed per frame
+    per_frame = {k: sum(v) / args.steps for k, v in acc.items()}
+    per_launch = {k: float(np.mean(v)) for k, v in acc.items()}
+    dominant = max(per_frame, key=per_frame.get)
+
+    # ---- workload statistics ----
+    tri_per_instance = scene.instance_triangles()
+    _, _, vis0 = r.view.prefix(0)
+    visible_candidates = int(tri_per_instance[vis0["index"].astype(np.int64)].sum())
+    counters = r.view.counters()
+    vis = r.view.visbuffer()
+    covered = int((vis != np.uint64(abi.LH_VISBUFFER_CLEAR)).sum())
+    sizes = meshlet_sizes(scene)
+
+    roofline = None
+    if dominant.startswith("raster_meshlets"):
+        if dominant.endswith("_view"):
+            algo = raster_algorithmic_bytes(scene, r.view.draws(0), covered, sizes)
+        else:
+            shadow_maps = r.view.shadow_maps()
+            algo = 0
+            for c in range(abi.NUM_CASCADES):
+                cov_c = int((shadow_maps[c] != np.uint32(abi.LH_SHADOW_CLEAR)).sum())
+                algo += raster_algorithmic_bytes(scene, r.view.draws(1 + c), cov_c, sizes) - 4 * cov_c
+            algo //= abi.NUM_CASCADES
+        achieved = algo / (per_launch[dominant] * 1e-3) / 1e9
+        traffic = None
+        tfile = ROOT / "profiles" / "traffic.json"
+        if tfile.exists():
+            traffic = json.loads(tfile.read_text()).get(dominant)
+        roofline = {"kernel": dominant, "bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
+                    "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
+                    "algorithmic_bytes": algo, "avg_launch_ms": round(per_launch[dominant], 5)}
+    else:
+        roofline = {"kernel": dominant, "bound": "hbm", "achieved": None, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                    "frac": None, "traffic": None, "avg_launch_ms": round(per_launch[dominant], 5)}
+
+    batched = None
+    if args.batch_steps > 0 and args.workload == "large":
+        del r
+        torch.cuda.empty_cache()
+        batched = batch_bench(scene, args.batch_steps)
+
+    cpu = None
+    if args.cpu_frames > 0:
+        cpu = cpu_baseline(scene, cfg, make_uniforms(cam, scene.num_instances), args.cpu_frames)
+
+    out = {
+        "metric": METRIC,
+        "value": round(ms, 4),
+        "unit": "ms/frame",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms, 4),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32+i64",
+        "data": "synthetic (seeded procedural scene)",
+        "gtri_per_s": round(visible_candidates / (ms * 1e-3) / 1e9, 3),
+        "frames_per_s": round(1000.0 / ms, 1),
+        "config": {
+            "workload": f"plain frame, {args.workload}",
+            "instances": scene.num_instances,
+            "meshes": scene.num_meshes,
+            "scene_triangles": int(scene.total_triangles),
+            "visible_candidate_triangles": visible_candidates,
+            "resolution": f"{cfg.width}x{cfg.height}",
+            "cascades": f"4x{shadow}^2",
+            "surviving_meshlets_per_pass": [int(x) for x in counters[:5]],
+            "covered_pixels": covered,
+            "queued_triangles_view_shadow": queue_counts,
+            "tuning": {str(c): _tuning_of(c) for c in range(5)},
+        },
+        "kernel_ms_per_frame": {k: round(v, 4) for k, v in sorted(per_frame.items(), key=lambda kv: -kv[1])},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "batched_views": batched,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

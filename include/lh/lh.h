@@ -42,6 +42,8 @@ typedef void* lh_stream; /* hipStream_t */
 #define LH_NUM_PASSES 5
 #define LH_NUM_CASCADES 4
 #define LH_MAX_VIEW_SIDE 8192u
+#define LH_MAX_BATCH_VIEWS 256u
+#define LH_MAX_BATCH_SEGMENTS (LH_MAX_BATCH_VIEWS * LH_NUM_CASCADES)
 
 /* Device counters of one view (u32 each). */
 #define LH_CNT_DRAWS 0     /* + pass: surviving meshlets of the pass (may exceed draw_capacity) */
@@ -58,7 +60,7 @@ typedef void* lh_stream; /* hipStream_t */
 
 /* Bytes per entry of the two raster queues inside lh_view.scratch:
  * [clip queue: clip_capacity x 4 B][pad to 256][big queue: big_capacity x 48 B]. */
-#define LH_CLIP_ENTRY_BYTES 4u
+#define LH_CLIP_ENTRY_BYTES 8u
 #define LH_BIG_ENTRY_BYTES 48u
 
 typedef struct lh_scene {
@@ -147,6 +149,38 @@ lh_status lh_tonemap(const lh_view* view, lh_stream stream);
 /* The whole frame in the reference's pass order (src/rendering.cpp:79-502). */
 lh_status lh_frame(const lh_scene* scene, const lh_view* view, lh_stream stream);
 
+/* lh_frame with per-kernel device times: an event is recorded on `stream`
+ * before every kernel launch, the call synchronizes on the last one and
+ * writes one entry per kernel (in launch order).  No counterpart in the
+ * reference (its per-pass GPU zones are Tracy's, src/rendering.cpp:42-512). */
+typedef struct lh_kernel_time {
+    char name[40];
+    float ms;
+} lh_kernel_time;
+lh_status lh_frame_profiled(const lh_scene* scene, const lh_view* view, lh_stream stream, lh_kernel_time* out,
+                            uint32_t capacity, uint32_t* count);
+
+/* ---- batched views (no counterpart in the reference, which renders one view
+ * per frame, src/rendering.cpp:28-548) ----
+ * A batch is a fixed set of up to LH_MAX_BATCH_VIEWS views over one scene,
+ * each view a camera with its own four cascades and buffers (caller-owned, as
+ * for lh_frame).  lh_batch_create copies the view table to the device (the
+ * only allocation; nothing is allocated per frame).  lh_batch_run executes one
+ * frame of every view either as the per-view loop (LH_BATCH_LOOP: lh_frame per
+ * view, in order) or batched (LH_BATCH_BATCHED: each pass launched once for
+ * all views, flattened, load-balanced grids); with use_graph != 0 the sequence
+ * is captured into a hipGraph on first use and replayed.  Both modes produce
+ * bit-identical keys, shadow depths and radiance for every view. */
+typedef struct lh_batch lh_batch;
+#define LH_BATCH_LOOP 0u
+#define LH_BATCH_BATCHED 1u
+lh_status lh_batch_create(const lh_scene* scene, const lh_view* views, uint32_t n_views, lh_batch** out);
+lh_status lh_batch_run(lh_batch* batch, uint32_t mode, uint32_t use_graph, lh_stream stream);
+/* One un-captured frame of the batch with per-kernel device times (see lh_frame_profiled). */
+lh_status lh_batch_profiled(lh_batch* batch, uint32_t mode, lh_stream stream, lh_kernel_time* out,
+                            uint32_t capacity, uint32_t* count);
+void lh_batch_destroy(lh_batch* batch);
+
 /* Copies the view's counters to host memory (synchronizes the stream) and
  * returns LH_ERR_CAPACITY if any overflow bit is set. */
 lh_status lh_view_status(const lh_view* view, uint32_t* host_counters, lh_stream stream);
@@ -155,15 +189,24 @@ lh_status lh_view_status(const lh_view* view, uint32_t* host_counters, lh_stream
  * reference, whose raster is fixed-function).  Results are independent of
  * them; the autotuner picks them per resolution class.  Process-global. */
 typedef struct lh_tuning {
-    uint32_t small_extent_px; /* triangles whose vertex bbox fits in this many pixels (1..32) are
-                                 rasterized by one lane; larger ones cooperatively */
-    uint32_t meshlet_blocks;  /* persistent grid of the per-meshlet raster (one wave per block) */
-    uint32_t big_blocks;      /* persistent grid of the large-triangle raster */
-    uint32_t clip_blocks;     /* persistent grid of the clipping kernel */
-    uint32_t cull_blocks;     /* persistent grid of the meshlet cull */
+    uint32_t small_extent_px;  /* triangles whose vertex bbox fits in this many pixels (1..32) are
+                                  rasterized by one lane */
+    uint32_t meshlet_blocks;   /* persistent grid of the per-meshlet raster (one wave per block) */
+    uint32_t big_blocks;       /* persistent grid of the large-triangle raster */
+    uint32_t clip_blocks;      /* persistent grid of the clipping kernel */
+    uint32_t cull_blocks;      /* persistent grid of the meshlet cull */
+    uint32_t medium_extent_px; /* up to this extent (small_extent_px..64) the whole wave rasterizes
+                                  the triangle in place; larger ones go to the big-triangle queue */
 } lh_tuning;
+/* Sets / reads every class at once (lh_set_tuning) or one class.  Classes by
+ * main-view pixels: 0 < 0.6 MP, 1 < 2.5 MP, 2 < 9 MP, 3 larger; class 4 is used
+ * by batched launches (lh_batch_run with LH_BATCH_BATCHED). */
+#define LH_TUNING_CLASSES 5
 lh_status lh_set_tuning(const lh_tuning* tuning);
 lh_status lh_get_tuning(lh_tuning* tuning);
+lh_status lh_set_tuning_class(uint32_t tuning_class, const lh_tuning* tuning);
+lh_status lh_get_tuning_class(uint32_t tuning_class, lh_tuning* tuning);
+uint32_t lh_tuning_class_of(uint32_t width, uint32_t height);
 
 /* Library version string. */
 const char* lh_version(void);

@@ -35,6 +35,7 @@ HIP_SOURCES = [
     "resolve.hip",
     "backward.hip",
     "capi.hip",
+    "batch.hip",
 ]
 
 # Device code: gfx950 only; no FMA contraction; IEEE div/sqrt; packed fp32

@@ -1,44 +1,49 @@
-// C ABI of liblh.so (include/lh/lh.h) and the frame recorder.
+// C ABI of liblh.so (include/lh/lh.h): single-view passes, the frame
+// recorder and its profiled variant.
 //
 // lh_frame issues the passes in the order of the reference's render()
 // (src/rendering.cpp:79-502): reset -> instance cull -> meshlet cull ->
-// visibility raster -> depth reduction -> cascade matrices -> shadow instance
-// cull -> per cascade (meshlet cull, shadow raster) -> resolve (+ display
-// transform).  Vulkan barriers become stream order; the reference's indirect
-// dispatch sizes (reset_buffers_b/c) become persistent grids that read the
-// device-side counts.
+// visibility raster -> depth reduction -> cascade matrices + shadow instance
+// cull -> meshlet cull of the four cascades -> shadow raster of the four
+// cascades -> resolve (+ display transform).  Vulkan barriers become stream
+// order; the reference's indirect dispatch sizes (reset_buffers_b/c) become
+// persistent grids that read the device-side counts.
 #include <cstring>
 
-#include "kernels.h"
+#include "capi_common.h"
+
+namespace lhk {
+Profiler* g_profiler = nullptr;
+Tuning g_tunings[LH_TUNING_CLASSES];
+}  // namespace lhk
 
 namespace {
 
-lhk::Tuning g_tuning;
+lhk::Profiler g_frame_profiler;
 
-lh_status hip_status(hipError_t e) { return e == hipSuccess ? LH_OK : LH_ERR_HIP; }
-
-bool valid_scene(const lh_scene* s) {
-    return s && (s->num_instances == 0 || (s->instances && s->instance_meshlet_base)) && s->mesh_infos &&
-           s->factors && s->meshlet_table && s->total_meshlets <= LH_MAX_SCENE_MESHLETS &&
-           (s->num_textures == 0 || s->textures);
-}
-
-bool valid_view(const lh_view* v) {
-    if (!v || !v->uniforms || !v->misc || !v->prefix_sums || !v->draws || !v->counters) return false;
-    if (v->width == 0 || v->height == 0 || v->width > LH_MAX_VIEW_SIDE || v->height > LH_MAX_VIEW_SIDE) return false;
-    if (v->shadow_size > LH_MAX_VIEW_SIDE) return false;
-    if (v->instance_capacity == 0 || v->draw_capacity == 0 || v->draw_capacity > (1u << 25)) return false;
-    return true;
-}
-
-bool valid_raster(const lh_view* v) {
-    return v->scratch && v->scratch_bytes >= lh_scratch_bytes(v->big_capacity, v->clip_capacity) &&
-           v->big_capacity > 0 && v->clip_capacity > 0;
-}
-
-hipStream_t stream_of(lh_stream s) { return static_cast<hipStream_t>(s); }
+lhd::OneView one(const lh_view* v) { return lhd::OneView{*v}; }
 
 }  // namespace
+
+namespace lhk {
+
+hipError_t record_frame(const lh_scene& s, const lh_view& v, hipStream_t st) {
+    const lhd::OneView vs{v};
+    const uint64_t px = uint64_t(v.width) * v.height;
+    hipError_t e = view_reset(vs, st);
+    if (e == hipSuccess) e = clear_targets(vs, std::max<uint64_t>(px, uint64_t(v.shadow_size) * v.shadow_size), st);
+    if (e == hipSuccess) e = cull_instances(s, vs, false, st);
+    if (e == hipSuccess) e = cull_meshlets(s, vs, 0, 1, tuning_for(v).cull_blocks, st);
+    if (e == hipSuccess) e = raster(s, vs, 0, 1, tuning_for(v).raster, st);
+    if (e == hipSuccess) e = depth_bounds(vs, px, st);
+    if (e == hipSuccess) e = fit_cull_instances_shadow(s, vs, st);
+    if (e == hipSuccess) e = cull_meshlets(s, vs, 1, LH_NUM_CASCADES, tuning_for(v).cull_blocks, st);
+    if (e == hipSuccess) e = raster(s, vs, 1, LH_NUM_CASCADES, tuning_for(v).raster, st);
+    if (e == hipSuccess) e = resolve(s, vs, 0, 0, st);
+    return e;
+}
+
+}  // namespace lhk
 
 extern "C" {
 
@@ -48,88 +53,83 @@ uint64_t lh_scratch_bytes(uint32_t big_capacity, uint32_t clip_capacity) {
 }
 
 lh_status lh_view_reset(const lh_scene* s, const lh_view* v, lh_stream stream) {
-    if (!valid_scene(s) || !valid_view(v)) return LH_ERR_INVALID;
-    hipStream_t st = stream_of(stream);
-    hipError_t e = lhk::view_reset(*v, st);
-    if (e == hipSuccess && v->visbuffer)
-        e = lhk::fill_u64(v->visbuffer, uint64_t(v->width) * v->height, LH_VISBUFFER_CLEAR, st);
-    if (e == hipSuccess && v->shadow_maps && v->shadow_size)
-        e = lhk::fill_u32(v->shadow_maps, uint64_t(LH_NUM_CASCADES) * v->shadow_size * v->shadow_size,
-                          LH_SHADOW_CLEAR, st);
-    return hip_status(e);
+    if (!lhc::valid_scene(s) || !lhc::valid_view(v)) return LH_ERR_INVALID;
+    hipStream_t st = lhc::stream_of(stream);
+    hipError_t e = lhk::view_reset(one(v), st);
+    if (e == hipSuccess && v->visbuffer && (v->shadow_maps || v->shadow_size == 0)) {
+        lh_view w = *v;
+        if (!w.shadow_maps) w.shadow_size = 0;
+        e = lhk::clear_targets(lhd::OneView{w},
+                               std::max<uint64_t>(uint64_t(w.width) * w.height, uint64_t(w.shadow_size) * w.shadow_size),
+                               st);
+    }
+    return lhc::hip_status(e);
 }
 
 lh_status lh_cull_instances(const lh_scene* s, const lh_view* v, lh_stream stream) {
-    if (!valid_scene(s) || !valid_view(v)) return LH_ERR_INVALID;
-    return hip_status(lhk::cull_instances(*s, *v, false, stream_of(stream)));
+    if (!lhc::valid_scene(s) || !lhc::valid_view(v)) return LH_ERR_INVALID;
+    return lhc::hip_status(lhk::cull_instances(*s, one(v), false, lhc::stream_of(stream)));
 }
 
 lh_status lh_cull_meshlets(const lh_scene* s, const lh_view* v, uint32_t pass, lh_stream stream) {
-    if (!valid_scene(s) || !valid_view(v) || pass >= LH_NUM_PASSES) return LH_ERR_INVALID;
-    return hip_status(lhk::cull_meshlets(*s, *v, pass, 1, g_tuning.cull_blocks, stream_of(stream)));
+    if (!lhc::valid_scene(s) || !lhc::valid_view(v) || pass >= LH_NUM_PASSES) return LH_ERR_INVALID;
+    return lhc::hip_status(lhk::cull_meshlets(*s, one(v), pass, 1, lhk::tuning_for(*v).cull_blocks, lhc::stream_of(stream)));
 }
 
 lh_status lh_raster_view(const lh_scene* s, const lh_view* v, lh_stream stream) {
-    if (!valid_scene(s) || !valid_view(v) || !v->visbuffer || !valid_raster(v)) return LH_ERR_INVALID;
-    return hip_status(lhk::raster(*s, *v, 0, g_tuning.raster, stream_of(stream)));
+    if (!lhc::valid_scene(s) || !lhc::valid_view(v) || !v->visbuffer || !lhc::valid_raster(v)) return LH_ERR_INVALID;
+    return lhc::hip_status(lhk::raster(*s, one(v), 0, 1, lhk::tuning_for(*v).raster, lhc::stream_of(stream)));
 }
 
 lh_status lh_depth_bounds(const lh_scene* s, const lh_view* v, lh_stream stream) {
-    if (!valid_scene(s) || !valid_view(v) || !v->visbuffer) return LH_ERR_INVALID;
-    return hip_status(lhk::depth_bounds(*v, stream_of(stream)));
+    if (!lhc::valid_scene(s) || !lhc::valid_view(v) || !v->visbuffer) return LH_ERR_INVALID;
+    return lhc::hip_status(lhk::depth_bounds(one(v), uint64_t(v->width) * v->height, lhc::stream_of(stream)));
 }
 
 lh_status lh_fit_cascades(const lh_scene* s, const lh_view* v, lh_stream stream) {
-    if (!valid_scene(s) || !valid_view(v)) return LH_ERR_INVALID;
-    return hip_status(lhk::fit_cascades(*v, stream_of(stream)));
+    if (!lhc::valid_scene(s) || !lhc::valid_view(v)) return LH_ERR_INVALID;
+    return lhc::hip_status(lhk::fit_cascades(one(v), lhc::stream_of(stream)));
 }
 
 lh_status lh_cull_instances_shadow(const lh_scene* s, const lh_view* v, lh_stream stream) {
-    if (!valid_scene(s) || !valid_view(v)) return LH_ERR_INVALID;
-    return hip_status(lhk::cull_instances(*s, *v, true, stream_of(stream)));
+    if (!lhc::valid_scene(s) || !lhc::valid_view(v)) return LH_ERR_INVALID;
+    return lhc::hip_status(lhk::cull_instances(*s, one(v), true, lhc::stream_of(stream)));
 }
 
 lh_status lh_raster_shadow(const lh_scene* s, const lh_view* v, uint32_t cascade, lh_stream stream) {
-    if (!valid_scene(s) || !valid_view(v) || cascade >= LH_NUM_CASCADES || !v->shadow_maps || v->shadow_size == 0 ||
-        !valid_raster(v))
+    if (!lhc::valid_scene(s) || !lhc::valid_view(v) || cascade >= LH_NUM_CASCADES || !v->shadow_maps ||
+        v->shadow_size == 0 || !lhc::valid_raster(v))
         return LH_ERR_INVALID;
-    return hip_status(lhk::raster(*s, *v, 1 + cascade, g_tuning.raster, stream_of(stream)));
+    return lhc::hip_status(lhk::raster(*s, one(v), 1 + cascade, 1, lhk::tuning_for(*v).raster, lhc::stream_of(stream)));
 }
 
 lh_status lh_resolve(const lh_scene* s, const lh_view* v, lh_stream stream) {
-    if (!valid_scene(s) || !valid_view(v) || !v->visbuffer || !v->radiance || !v->shadow_maps ||
+    if (!lhc::valid_scene(s) || !lhc::valid_view(v) || !v->visbuffer || !v->radiance || !v->shadow_maps ||
         v->shadow_size == 0)
         return LH_ERR_INVALID;
-    return hip_status(lhk::resolve(*s, *v, stream_of(stream)));
+    return lhc::hip_status(lhk::resolve(*s, one(v), 0, 0, lhc::stream_of(stream)));
 }
 
 lh_status lh_tonemap(const lh_view* v, lh_stream stream) {
     if (!v || !v->radiance || !v->ldr || v->width == 0 || v->height == 0) return LH_ERR_INVALID;
-    return hip_status(lhk::tonemap(*v, stream_of(stream)));
+    return lhc::hip_status(lhk::tonemap(one(v), uint64_t(v->width) * v->height, lhc::stream_of(stream)));
 }
 
 lh_status lh_frame(const lh_scene* s, const lh_view* v, lh_stream stream) {
-    if (!valid_scene(s) || !valid_view(v) || !v->visbuffer || !v->radiance || !v->shadow_maps ||
-        v->shadow_size == 0 || !valid_raster(v))
-        return LH_ERR_INVALID;
-    hipStream_t st = stream_of(stream);
-    lh_status r;
-    if ((r = lh_view_reset(s, v, stream))) return r;
-    hipError_t e = lhk::cull_instances(*s, *v, false, st);
-    if (e == hipSuccess) e = lhk::cull_meshlets(*s, *v, 0, 1, g_tuning.cull_blocks, st);
-    if (e == hipSuccess) e = lhk::raster(*s, *v, 0, g_tuning.raster, st);
-    if (e == hipSuccess) e = lhk::depth_bounds(*v, st);
-    if (e == hipSuccess) e = lhk::fit_cascades(*v, st);
-    if (e == hipSuccess) e = lhk::cull_instances(*s, *v, true, st);
-    if (e == hipSuccess) e = lhk::cull_meshlets(*s, *v, 1, LH_NUM_CASCADES, g_tuning.cull_blocks, st);
-    for (uint32_t c = 0; c < LH_NUM_CASCADES && e == hipSuccess; c++) e = lhk::raster(*s, *v, 1 + c, g_tuning.raster, st);
-    if (e == hipSuccess) e = lhk::resolve(*s, *v, st);
-    return hip_status(e);
+    if (!lhc::valid_frame(s, v)) return LH_ERR_INVALID;
+    return lhc::hip_status(lhk::record_frame(*s, *v, lhc::stream_of(stream)));
+}
+
+lh_status lh_frame_profiled(const lh_scene* s, const lh_view* v, lh_stream stream, lh_kernel_time* out,
+                            uint32_t capacity, uint32_t* count) {
+    if (!lhc::valid_frame(s, v) || !out || !count) return LH_ERR_INVALID;
+    return lhc::profile(g_frame_profiler, stream, out, capacity, count,
+                        [&](hipStream_t st) { return lhk::record_frame(*s, *v, st); });
 }
 
 lh_status lh_view_status(const lh_view* v, uint32_t* host_counters, lh_stream stream) {
     if (!v || !v->counters || !host_counters) return LH_ERR_INVALID;
-    hipStream_t st = stream_of(stream);
+    hipStream_t st = lhc::stream_of(stream);
     if (hipMemcpyAsync(host_counters, v->counters, LH_NUM_COUNTERS * sizeof(uint32_t), hipMemcpyDeviceToHost, st) !=
             hipSuccess ||
         hipStreamSynchronize(st) != hipSuccess)
@@ -137,28 +137,45 @@ lh_status lh_view_status(const lh_view* v, uint32_t* host_counters, lh_stream st
     return host_counters[LH_CNT_OVERFLOW] ? LH_ERR_CAPACITY : LH_OK;
 }
 
-lh_status lh_set_tuning(const lh_tuning* t) {
-    if (!t || t->small_extent_px == 0 || t->small_extent_px > 32 || t->meshlet_blocks == 0 || t->big_blocks == 0 ||
-        t->clip_blocks == 0 || t->cull_blocks == 0)
+lh_status lh_set_tuning_class(uint32_t cls, const lh_tuning* t) {
+    if (cls >= LH_TUNING_CLASSES || !t || t->small_extent_px == 0 || t->small_extent_px > 32 ||
+        t->meshlet_blocks == 0 || t->big_blocks == 0 || t->clip_blocks == 0 || t->cull_blocks == 0 ||
+        t->medium_extent_px < t->small_extent_px || t->medium_extent_px > 64)
         return LH_ERR_INVALID;
-    g_tuning.raster.small_extent_px = t->small_extent_px;
-    g_tuning.raster.meshlet_blocks = t->meshlet_blocks;
-    g_tuning.raster.big_blocks = t->big_blocks;
-    g_tuning.raster.clip_blocks = t->clip_blocks;
-    g_tuning.cull_blocks = t->cull_blocks;
+    lhk::Tuning& g = lhk::g_tunings[cls];
+    g.raster.small_extent_px = t->small_extent_px;
+    g.raster.medium_extent_px = t->medium_extent_px;
+    g.raster.meshlet_blocks = t->meshlet_blocks;
+    g.raster.big_blocks = t->big_blocks;
+    g.raster.clip_blocks = t->clip_blocks;
+    g.cull_blocks = t->cull_blocks;
     return LH_OK;
 }
 
-lh_status lh_get_tuning(lh_tuning* t) {
-    if (!t) return LH_ERR_INVALID;
-    t->small_extent_px = g_tuning.raster.small_extent_px;
-    t->meshlet_blocks = g_tuning.raster.meshlet_blocks;
-    t->big_blocks = g_tuning.raster.big_blocks;
-    t->clip_blocks = g_tuning.raster.clip_blocks;
-    t->cull_blocks = g_tuning.cull_blocks;
+lh_status lh_get_tuning_class(uint32_t cls, lh_tuning* t) {
+    if (cls >= LH_TUNING_CLASSES || !t) return LH_ERR_INVALID;
+    const lhk::Tuning& g = lhk::g_tunings[cls];
+    t->small_extent_px = g.raster.small_extent_px;
+    t->medium_extent_px = g.raster.medium_extent_px;
+    t->meshlet_blocks = g.raster.meshlet_blocks;
+    t->big_blocks = g.raster.big_blocks;
+    t->clip_blocks = g.raster.clip_blocks;
+    t->cull_blocks = g.cull_blocks;
     return LH_OK;
 }
 
-const char* lh_version(void) { return "lighthugger_amd 0.1 (gfx950)"; }
+lh_status lh_set_tuning(const lh_tuning* t) {
+    for (uint32_t c = 0; c < LH_TUNING_CLASSES; c++) {
+        lh_status s = lh_set_tuning_class(c, t);
+        if (s) return s;
+    }
+    return LH_OK;
+}
+
+lh_status lh_get_tuning(lh_tuning* t) { return lh_get_tuning_class(1, t); }
+
+uint32_t lh_tuning_class_of(uint32_t width, uint32_t height) { return lhk::tuning_class(width, height); }
+
+const char* lh_version(void) { return "lighthugger_amd 0.2 (gfx950)"; }
 
 }  // extern "C"

@@ -1,10 +1,11 @@
 // Culling passes of the GPU-driven frame (gfx950).
 //
-//   k_view_reset          reset_buffers_a    src/shaders/compute/reset_buffers.glsl:9-29
-//   k_fill_u64/u32        attachment clears  src/rendering.cpp:206-211, :384
-//   k_cull_instances      cull_instances     src/shaders/cull_instances.comp:7-27
-//                         cull_instances_shadows               :31-56
-//   k_cull_meshlets       write_draw_calls(_shadows)  src/shaders/write_draw_calls.comp:7-117
+//   k_view_reset              reset_buffers_a    src/shaders/compute/reset_buffers.glsl:9-29
+//   k_clear_targets           attachment clears  src/rendering.cpp:206-211, :384
+//   k_cull_instances          cull_instances     src/shaders/cull_instances.comp:7-27
+//                             cull_instances_shadows               :31-56
+//   k_fit_cull_instances_shadow  generate_shadow_matrices + cull_instances_shadows
+//   k_cull_meshlets           write_draw_calls(_shadows)  src/shaders/write_draw_calls.comp:7-117
 //
 // The prefix-sum append keeps the reference's single 64-bit atomic per append
 // (count in the high word, running sum in the low word,
@@ -12,8 +13,12 @@
 // one atomic: lanes take consecutive slots in lane order and their inclusive
 // sums from a wave scan, so the sector has exactly the invariants of
 // one-by-one appends (sums strictly increasing with the slot).
-#include "lh_device.h"
+//
+// Every kernel is templated on its view source (one view, or a table of views
+// indexed by blockIdx.z) so a batch of views runs the same arithmetic.
+#include "cascade.h"
 #include "kernels.h"
+#include "lh_device.h"
 
 using namespace lhd;
 
@@ -44,23 +49,24 @@ LH_DEV bool cull_sphere(const Instance& in, const LhVec4& s, const Uniforms& u) 
     return !visible;
 }
 
-LH_DEV f2 shadow_view_xy(const MiscStorage& m, uint32_t c, f3 w) {
-    f3 v = mat4_point(m.shadow_view_matrices[c], w);
+LH_DEV f2 shadow_view_xy(const LhMat4& view, f3 w) {
+    f3 v = mat4_point(view, w);
     return {v.x, v.y};
 }
 
 // cull_bounding_sphere_shadows (culling.glsl:82-135) with
 // fits_partially_inside / fits_entirely_inside(_cascade) (:48-80).
-LH_DEV bool cull_sphere_shadow(const Instance& in, const LhVec4& s, const MiscStorage& m, uint32_t c) {
+LH_DEV bool cull_sphere_shadow(const Instance& in, const LhVec4& s, const LhMat4* views, const float* radii,
+                               uint32_t c) {
     f3 w;
     float r;
     world_sphere(in, s, w, r);
-    f2 p = shadow_view_xy(m, c, w);
-    float R = m.shadow_sphere_radii[c];
+    f2 p = shadow_view_xy(views[c], w);
+    float R = radii[c];
     if (!((fabsf(p.x) - r < R) && (fabsf(p.y) - r < R))) return true;
     for (uint32_t k = 0; k < c; k++) {
-        f2 q = shadow_view_xy(m, k, w);
-        float Rk = m.shadow_sphere_radii[k];
+        f2 q = shadow_view_xy(views[k], w);
+        float Rk = radii[k];
         if ((fabsf(q.x) + r < Rk) && (fabsf(q.y) + r < Rk)) return true;
     }
     return false;
@@ -119,7 +125,9 @@ LH_DEV bool prefix_search(const uint64_t* sec, uint32_t n, uint32_t target, Pref
 
 }  // namespace
 
-__global__ void __launch_bounds__(64) k_view_reset(lh_view v) {
+template <class VS>
+__global__ void __launch_bounds__(64) k_view_reset(VS vs) {
+    const lh_view& v = vs.get(blockIdx.z);
     uint32_t t = threadIdx.x;
     v.counters[t] = 0;  // LH_NUM_COUNTERS == 64
     if (t < LH_NUM_PASSES) sector(v, t)[0] = 0;
@@ -129,27 +137,28 @@ __global__ void __launch_bounds__(64) k_view_reset(lh_view v) {
     }
 }
 
-__global__ void __launch_bounds__(256) k_fill_u64(uint64_t* p, uint64_t n, uint64_t value) {
-    // Two keys per store (16 B); n is even for every caller except the tail.
-    uint64_t i = (uint64_t(blockIdx.x) * blockDim.x + threadIdx.x) * 2;
-    uint64_t stride = uint64_t(gridDim.x) * blockDim.x * 2;
-    for (; i + 1 < n; i += stride) {
-        ulonglong2 val = make_ulonglong2(value, value);
-        *reinterpret_cast<ulonglong2*>(p + i) = val;
-    }
-    if (i < n) p[i] = value;
-}
-
-__global__ void __launch_bounds__(256) k_fill_u32(uint32_t* p, uint64_t n, uint32_t value) {
-    uint64_t i = (uint64_t(blockIdx.x) * blockDim.x + threadIdx.x) * 4;
-    uint64_t stride = uint64_t(gridDim.x) * blockDim.x * 4;
-    for (; i + 3 < n; i += stride) *reinterpret_cast<uint4*>(p + i) = make_uint4(value, value, value, value);
-    for (; i < n; i++) p[i] = value;
+// Clears the visibility buffer (two keys per 16-byte store) and the shadow
+// maps (four texels per store) of view blockIdx.z with plain vector stores.
+template <class VS>
+__global__ void __launch_bounds__(256) k_clear_targets(VS vs) {
+    const lh_view& v = vs.get(blockIdx.z);
+    const uint64_t nv = uint64_t(v.width) * v.height;
+    const uint64_t ns = v.shadow_maps ? uint64_t(LH_NUM_CASCADES) * v.shadow_size * v.shadow_size : 0;
+    const uint64_t stride = uint64_t(gridDim.x) * blockDim.x;
+    const uint64_t t0 = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    const ulonglong2 kv = make_ulonglong2(LH_VISBUFFER_CLEAR, LH_VISBUFFER_CLEAR);
+    for (uint64_t i = t0; 2 * i + 1 < nv; i += stride) reinterpret_cast<ulonglong2*>(v.visbuffer)[i] = kv;
+    if (t0 == 0 && (nv & 1)) v.visbuffer[nv - 1] = LH_VISBUFFER_CLEAR;
+    const uint4 ks = make_uint4(LH_SHADOW_CLEAR, LH_SHADOW_CLEAR, LH_SHADOW_CLEAR, LH_SHADOW_CLEAR);
+    for (uint64_t i = t0; 4 * i + 3 < ns; i += stride) reinterpret_cast<uint4*>(v.shadow_maps)[i] = ks;
+    if (t0 == 0)
+        for (uint64_t i = ns & ~uint64_t(3); i < ns; i++) v.shadow_maps[i] = LH_SHADOW_CLEAR;
 }
 
 // Main view (shadow = false, sector 0) or cascade blockIdx.y (sector 1 + c).
-template <bool SHADOW>
-__global__ void __launch_bounds__(256) k_cull_instances(lh_scene s, lh_view v) {
+template <bool SHADOW, class VS>
+__global__ void __launch_bounds__(256) k_cull_instances(lh_scene s, VS vs) {
+    const lh_view& v = vs.get(blockIdx.z);
     const Uniforms& u = *v.uniforms;
     uint32_t n = min(u.num_instances, s.num_instances);
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -160,15 +169,52 @@ __global__ void __launch_bounds__(256) k_cull_instances(lh_scene s, lh_view v) {
         const Instance& in = s.instances[i];
         const MeshInfo& mi = mesh_info_at(in.mesh_info_address);
         value = mi.num_meshlets;
-        visible = SHADOW ? !cull_sphere_shadow(in, mi.bounding_sphere, *v.misc, cascade)
+        visible = SHADOW ? !cull_sphere_shadow(in, mi.bounding_sphere, v.misc->shadow_view_matrices,
+                                               v.misc->shadow_sphere_radii, cascade)
                          : !cull_sphere(in, mi.bounding_sphere, u);
     }
     wave_prefix_append(v, SHADOW ? 1 + cascade : 0, visible, i, value);
 }
 
-// One pass per blockIdx.y (pass = pass0 + blockIdx.y); grid-stride over the
-// meshlets of the pass's visible instances.
-__global__ void __launch_bounds__(256) k_cull_meshlets(lh_scene s, lh_view v, uint32_t pass0) {
+// generate_shadow_matrices fused into cull_instances_shadows: every block
+// fits the four cascades itself (lanes 0..3, identical arithmetic, so every
+// block sees the same bits) into LDS and block (0, 0) publishes them to
+// MiscStorage; this saves one dependent launch per frame.
+template <class VS>
+__global__ void __launch_bounds__(256) k_fit_cull_instances_shadow(lh_scene s, VS vs) {
+    const lh_view& v = vs.get(blockIdx.z);
+    __shared__ LhMat4 views[LH_NUM_CASCADES];
+    __shared__ float radii[LH_NUM_CASCADES];
+    const Uniforms& u = *v.uniforms;
+    const uint32_t min_bits = v.misc->min_depth, max_bits = v.misc->max_depth;
+    if (threadIdx.x < LH_NUM_CASCADES) {
+        const cascade::Fit f = cascade::fit(u, min_bits, max_bits, threadIdx.x);
+        views[threadIdx.x] = f.view;
+        radii[threadIdx.x] = f.radius;
+        if (blockIdx.x == 0 && blockIdx.y == 0) cascade::store(*v.misc, threadIdx.x, f);
+    }
+    if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0 && min_bits == 0xFFFFFFFFu)
+        atomicOr(&v.counters[LH_CNT_FLAGS], LH_FLAG_EMPTY_VIEW);
+    __syncthreads();
+    const uint32_t n = min(u.num_instances, s.num_instances);
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t c = blockIdx.y;
+    bool visible = false;
+    uint32_t value = 0;
+    if (i < n) {
+        const Instance& in = s.instances[i];
+        const MeshInfo& mi = mesh_info_at(in.mesh_info_address);
+        value = mi.num_meshlets;
+        visible = !cull_sphere_shadow(in, mi.bounding_sphere, views, radii, c);
+    }
+    wave_prefix_append(v, 1 + c, visible, i, value);
+}
+
+// One pass per blockIdx.y (pass = pass0 + blockIdx.y), one view per
+// blockIdx.z; grid-stride over the meshlets of the pass's visible instances.
+template <class VS>
+__global__ void __launch_bounds__(256) k_cull_meshlets(lh_scene s, VS vs, uint32_t pass0) {
+    const lh_view& v = vs.get(blockIdx.z);
     const uint32_t pass = pass0 + blockIdx.y;
     const Uniforms& u = *v.uniforms;
     const uint64_t* sec = sector(v, pass);
@@ -191,7 +237,8 @@ __global__ void __launch_bounds__(256) k_cull_meshlets(lh_scene s, lh_view v, ui
             if (pass == 0) {
                 survive = !cull_sphere(in, ml.bounding_sphere, u) && (alpha || !cull_cone(in, ml, u, false));
             } else {
-                survive = !cull_sphere_shadow(in, ml.bounding_sphere, *v.misc, pass - 1) &&
+                survive = !cull_sphere_shadow(in, ml.bounding_sphere, v.misc->shadow_view_matrices,
+                                              v.misc->shadow_sphere_radii, pass - 1) &&
                           (alpha || !cull_cone(in, ml, u, true));
             }
         }
@@ -213,40 +260,61 @@ __global__ void __launch_bounds__(256) k_cull_meshlets(lh_scene s, lh_view v, ui
 
 namespace lhk {
 
-hipError_t view_reset(const lh_view& v, hipStream_t st) {
-    hipLaunchKernelGGL(k_view_reset, dim3(1), dim3(64), 0, st, v);
+template <class VS>
+hipError_t view_reset(const VS& vs, hipStream_t st) {
+    mark("view_reset", st);
+    hipLaunchKernelGGL(k_view_reset<VS>, dim3(1, 1, vs.count()), dim3(64), 0, st, vs);
     return hipGetLastError();
 }
 
-hipError_t fill_u64(uint64_t* p, uint64_t n, uint64_t value, hipStream_t st) {
-    if (n == 0) return hipSuccess;
-    uint64_t blocks = std::min<uint64_t>((n / 2 + 255) / 256 + 1, 8192);
-    hipLaunchKernelGGL(k_fill_u64, dim3(uint32_t(blocks)), dim3(256), 0, st, p, n, value);
+template <class VS>
+hipError_t clear_targets(const VS& vs, uint64_t max_pixels, hipStream_t st) {
+    // enough 256-thread blocks for ~8 stores per thread on the largest view
+    uint32_t blocks = uint32_t(std::min<uint64_t>(std::max<uint64_t>((max_pixels * 5 / 4 + 4095) / 4096, 1), 2048));
+    if (vs.count() > 1) blocks = std::max(blocks / 4, 16u);
+    mark("clear_targets", st);
+    hipLaunchKernelGGL(k_clear_targets<VS>, dim3(blocks, 1, vs.count()), dim3(256), 0, st, vs);
     return hipGetLastError();
 }
 
-hipError_t fill_u32(uint32_t* p, uint64_t n, uint32_t value, hipStream_t st) {
-    if (n == 0) return hipSuccess;
-    uint64_t blocks = std::min<uint64_t>((n / 4 + 255) / 256 + 1, 8192);
-    hipLaunchKernelGGL(k_fill_u32, dim3(uint32_t(blocks)), dim3(256), 0, st, p, n, value);
-    return hipGetLastError();
-}
-
-hipError_t cull_instances(const lh_scene& s, const lh_view& v, bool shadow, hipStream_t st) {
-    uint32_t n = s.num_instances;
-    if (n == 0) return hipSuccess;
-    dim3 grid((n + 255) / 256, shadow ? LH_NUM_CASCADES : 1);
+template <class VS>
+hipError_t cull_instances(const lh_scene& s, const VS& vs, bool shadow, hipStream_t st) {
+    uint32_t n = std::max(s.num_instances, 1u);
+    dim3 grid((n + 255) / 256, shadow ? LH_NUM_CASCADES : 1, vs.count());
+    mark(shadow ? "cull_instances_shadow" : "cull_instances", st);
     if (shadow)
-        hipLaunchKernelGGL(k_cull_instances<true>, grid, dim3(256), 0, st, s, v);
+        hipLaunchKernelGGL((k_cull_instances<true, VS>), grid, dim3(256), 0, st, s, vs);
     else
-        hipLaunchKernelGGL(k_cull_instances<false>, grid, dim3(256), 0, st, s, v);
+        hipLaunchKernelGGL((k_cull_instances<false, VS>), grid, dim3(256), 0, st, s, vs);
     return hipGetLastError();
 }
 
-hipError_t cull_meshlets(const lh_scene& s, const lh_view& v, uint32_t pass0, uint32_t npasses, uint32_t blocks,
-                         hipStream_t st) {
-    hipLaunchKernelGGL(k_cull_meshlets, dim3(blocks, npasses), dim3(256), 0, st, s, v, pass0);
+template <class VS>
+hipError_t fit_cull_instances_shadow(const lh_scene& s, const VS& vs, hipStream_t st) {
+    uint32_t n = std::max(s.num_instances, 1u);
+    mark("fit_cull_instances_shadow", st);
+    hipLaunchKernelGGL(k_fit_cull_instances_shadow<VS>, dim3((n + 255) / 256, LH_NUM_CASCADES, vs.count()),
+                       dim3(256), 0, st, s, vs);
     return hipGetLastError();
 }
+
+template <class VS>
+hipError_t cull_meshlets(const lh_scene& s, const VS& vs, uint32_t pass0, uint32_t npasses, uint32_t blocks,
+                         hipStream_t st) {
+    if (vs.count() > 1) blocks = std::max(blocks / 8, 32u);
+    mark(pass0 == 0 ? "cull_meshlets_view" : "cull_meshlets_shadow", st);
+    hipLaunchKernelGGL(k_cull_meshlets<VS>, dim3(blocks, npasses, vs.count()), dim3(256), 0, st, s, vs, pass0);
+    return hipGetLastError();
+}
+
+#define LH_INSTANTIATE(VS)                                                                                         \
+    template hipError_t view_reset<VS>(const VS&, hipStream_t);                                                    \
+    template hipError_t clear_targets<VS>(const VS&, uint64_t, hipStream_t);                                       \
+    template hipError_t cull_instances<VS>(const lh_scene&, const VS&, bool, hipStream_t);                        \
+    template hipError_t fit_cull_instances_shadow<VS>(const lh_scene&, const VS&, hipStream_t);                   \
+    template hipError_t cull_meshlets<VS>(const lh_scene&, const VS&, uint32_t, uint32_t, uint32_t, hipStream_t);
+LH_INSTANTIATE(lhd::OneView)
+LH_INSTANTIATE(lhd::ViewTable)
+#undef LH_INSTANTIATE
 
 }  // namespace lhk

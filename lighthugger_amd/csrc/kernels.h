@@ -1,4 +1,6 @@
-// Internal launcher interface between the C ABI (capi.hip) and the kernels.
+// Internal launcher interface between the C ABI (capi.hip, batch.hip) and the
+// kernels.  Launchers are templated on the view source (lhd::OneView for one
+// view, lhd::ViewTable for a batch) and instantiated for both.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -6,13 +8,15 @@
 #include <algorithm>
 
 #include "lh/lh.h"
+#include "lh_device.h"
 
 namespace lhk {
 
 // Raster tunables (chosen by the autotuner; output is independent of them).
 struct RasterTuning {
-    uint32_t small_extent_px = 8;   // in-lane raster when the vertex bbox is within this many pixels
-    uint32_t meshlet_blocks = 4096; // persistent grid of the per-meshlet kernel (64-thread blocks)
+    uint32_t small_extent_px = 4;   // in-lane raster when the vertex bbox is within this many pixels
+    uint32_t medium_extent_px = 64; // wave-cooperative raster up to this extent
+    uint32_t meshlet_blocks = 8192; // persistent grid of the per-meshlet kernel (64-thread blocks)
     uint32_t big_blocks = 2048;     // persistent grid of the large-triangle kernel (256-thread blocks)
     uint32_t clip_blocks = 256;     // persistent grid of the clip kernel (64-thread blocks)
 };
@@ -22,16 +26,53 @@ struct Tuning {
     uint32_t cull_blocks = 1024;    // persistent grid of the meshlet cull (256-thread blocks)
 };
 
-hipError_t view_reset(const lh_view& v, hipStream_t st);
-hipError_t fill_u64(uint64_t* p, uint64_t n, uint64_t value, hipStream_t st);
-hipError_t fill_u32(uint32_t* p, uint64_t n, uint32_t value, hipStream_t st);
-hipError_t cull_instances(const lh_scene& s, const lh_view& v, bool shadow, hipStream_t st);
-hipError_t cull_meshlets(const lh_scene& s, const lh_view& v, uint32_t pass0, uint32_t npasses, uint32_t blocks,
+// Optional per-kernel timing: when a Profiler is installed, every launcher
+// records an event on the launch stream right before each kernel, so the
+// interval between consecutive events is that kernel's device time.
+struct Profiler {
+    static constexpr int kMax = 2048;
+    hipEvent_t ev[kMax];
+    const char* name[kMax];
+    int n = 0;
+    bool created = false;
+};
+extern Profiler* g_profiler;
+inline void mark(const char* name, hipStream_t st) {
+    Profiler* p = g_profiler;
+    if (p && p->n < Profiler::kMax) {
+        (void)hipEventRecord(p->ev[p->n], st);  // a failed record shows up as a failed elapsed-time query
+        p->name[p->n++] = name;
+    }
+}
+
+template <class VS>
+hipError_t view_reset(const VS& vs, hipStream_t st);
+// Clears visibility buffers and shadow maps; max_pixels = largest W*H of the views.
+template <class VS>
+hipError_t clear_targets(const VS& vs, uint64_t max_pixels, hipStream_t st);
+template <class VS>
+hipError_t cull_instances(const lh_scene& s, const VS& vs, bool shadow, hipStream_t st);
+// generate_shadow_matrices + cull_instances_shadows in one launch.
+template <class VS>
+hipError_t fit_cull_instances_shadow(const lh_scene& s, const VS& vs, hipStream_t st);
+template <class VS>
+hipError_t cull_meshlets(const lh_scene& s, const VS& vs, uint32_t pass0, uint32_t npasses, uint32_t blocks,
                          hipStream_t st);
-hipError_t raster(const lh_scene& s, const lh_view& v, uint32_t pass, const RasterTuning& t, hipStream_t st);
-hipError_t depth_bounds(const lh_view& v, hipStream_t st);
-hipError_t fit_cascades(const lh_view& v, hipStream_t st);
-hipError_t resolve(const lh_scene& s, const lh_view& v, hipStream_t st);
-hipError_t tonemap(const lh_view& v, hipStream_t st);
+// Rasterizes passes [pass0, pass0 + npasses) in one set of launches (the main
+// view alone, or any run of cascades).
+template <class VS>
+hipError_t raster(const lh_scene& s, const VS& vs, uint32_t pass0, uint32_t npasses, const RasterTuning& t,
+                  hipStream_t st);
+template <class VS>
+hipError_t depth_bounds(const VS& vs, uint64_t max_pixels, hipStream_t st);
+template <class VS>
+hipError_t fit_cascades(const VS& vs, hipStream_t st);
+template <class VS>
+hipError_t resolve(const lh_scene& s, const VS& vs, uint32_t max_w, uint32_t max_h, hipStream_t st);
+template <class VS>
+hipError_t tonemap(const VS& vs, uint64_t max_pixels, hipStream_t st);
+
+// Batch plan: exclusive scan of the per-(view, pass) surviving meshlet counts.
+hipError_t plan_segments(const lhd::ViewTable& vt, uint32_t pass0, uint32_t npasses, hipStream_t st);
 
 }  // namespace lhk

@@ -16,6 +16,26 @@
 
 #define LH_DEV __device__ __forceinline__
 
+// Occupancy targets (waves per SIMD) of the two register-heavy kernels; a
+// build may override them.  0 leaves the register budget to the compiler.
+#ifndef LH_RASTER_WAVES
+#define LH_RASTER_WAVES 8
+#endif
+#ifndef LH_RESOLVE_WAVES
+#define LH_RESOLVE_WAVES 0
+#endif
+#define LH_WAVES_ATTR(n) __attribute__((amdgpu_waves_per_eu(n, 8)))
+#if LH_RASTER_WAVES > 0
+#define LH_RASTER_OCCUPANCY LH_WAVES_ATTR(LH_RASTER_WAVES)
+#else
+#define LH_RASTER_OCCUPANCY
+#endif
+#if LH_RESOLVE_WAVES > 0
+#define LH_RESOLVE_OCCUPANCY LH_WAVES_ATTR(LH_RESOLVE_WAVES)
+#else
+#define LH_RESOLVE_OCCUPANCY
+#endif
+
 namespace lhd {
 
 struct f2 {
@@ -124,5 +144,24 @@ LH_DEV const T* ptr(uint64_t addr) {
 LH_DEV uint64_t* sector(const lh_view& v, uint32_t pass) {
     return v.prefix_sums + size_t(pass) * (1 + v.instance_capacity);
 }
+
+// View sources: every kernel takes one of these and reads its view with
+// get(index).  OneView carries a single view by value (index ignored);
+// ViewTable reads entry `index` of a device-resident lh_view array, which is
+// how one launch covers every view of a batch with the same arithmetic.
+struct OneView {
+    lh_view v;
+    uint32_t count() const { return 1; }
+    LH_DEV const lh_view& get(uint32_t) const { return v; }
+};
+struct ViewTable {
+    const lh_view* views;
+    uint32_t n;
+    // Per-batch plan arrays (device): flattened raster segments and resolve tiles.
+    uint32_t* seg_offsets;        // [LH_NUM_PASSES * n + 1]
+    const uint32_t* tile_offsets; // [n + 1] resolve tiles (8x8) per view, exclusive scan
+    uint32_t count() const { return n; }
+    LH_DEV const lh_view& get(uint32_t i) const { return views[i]; }
+};
 
 }  // namespace lhd

@@ -16,14 +16,17 @@
 // Main view: 64-bit atomicMin of ~bits(z) << 32 | meshlet << 7 | tri (nearest
 // wins under reverse-Z); shadows: 32-bit atomicMin of bits(z).
 //
-// Work split for one pass (all persistent, grid-stride, stream ordered):
+// One launch covers a pass set - the main view, or all four cascades - whose
+// draw lists are concatenated into one index space, so the persistent grid is
+// balanced across cascades of very different load.  Work split:
 //   k_raster_meshlets  one wave per surviving meshlet: its <= 64 vertices are
 //                      transformed once (one per lane) into LDS, then each lane
 //                      sets up <= 2 of its <= 124 triangles.  Triangles whose
 //                      vertex bbox fits in `small_extent` pixels are rasterized
 //                      in the lane with int32 edge functions; larger ones go to
 //                      the big queue, clip-crossing ones to the clip queue
-//                      (both appended with one atomic per wave).
+//                      (both appended with one atomic per wave, tagged with
+//                      the pass).
 //   k_raster_clip      clips queued triangles and queues the fan pieces.
 //   k_raster_big       one 256-thread block per queued triangle, 16x16 tiles.
 #include "lh_device.h"
@@ -41,7 +44,7 @@ struct BigTri {
     int32_t Y[3];
     float z[3];
     uint32_t id;
-    uint16_t xlo, ylo, xhi, yhi;
+    uint16_t xlo, ylo, xhi, yhi;  // xlo bits 13..15: pass index within the set
 };
 static_assert(sizeof(BigTri) == LH_BIG_ENTRY_BYTES, "big queue entry");
 
@@ -50,16 +53,51 @@ struct SV {
     float z;
 };
 
+// One render target of a launch.
 struct Pass {
     uint32_t pass, W, H;
     uint64_t* vis;
     uint32_t* depth;
     const MeshletReference* draws;
-    uint32_t* clip_queue;
-    BigTri* big_queue;
-    uint32_t clip_cap, big_cap;
-    int32_t small_extent;  // subpixels
 };
+
+// A launch rasterizes passes [pass0, pass0 + n) of every view of its source;
+// the passes of one view share that view's queues, counted at
+// LH_CNT_CLIP / LH_CNT_BIG + pass0.
+struct PassSet {
+    uint32_t pass0, n;
+    int32_t small_extent;   // subpixels: in-lane raster up to this vertex-bbox extent
+    int32_t medium_extent;  // subpixels: wave-cooperative raster up to this extent (<= 64 px)
+};
+
+LH_DEV Pass make_pass(const lh_view& v, uint32_t pass) {
+    Pass p;
+    p.pass = pass;
+    if (pass == 0) {
+        p.W = v.width;
+        p.H = v.height;
+        p.vis = v.visbuffer;
+        p.depth = nullptr;
+    } else {
+        p.W = p.H = v.shadow_size;
+        p.vis = nullptr;
+        p.depth = v.shadow_maps + size_t(pass - 1) * v.shadow_size * v.shadow_size;
+    }
+    p.draws = v.draws + size_t(pass) * v.draw_capacity;
+    return p;
+}
+
+// Queue layout inside lh_view.scratch: clip entries {draw << 7 | triangle,
+// pass index within the set}, then (256-byte aligned) the big-triangle entries.
+struct Queues {
+    uint2* clip;
+    BigTri* big;
+};
+LH_DEV Queues queues(const lh_view& v) {
+    const uint64_t clip_bytes = (uint64_t(v.clip_capacity) * LH_CLIP_ENTRY_BYTES + 255) & ~uint64_t(255);
+    return {static_cast<uint2*>(v.scratch),
+            reinterpret_cast<BigTri*>(static_cast<char*>(v.scratch) + clip_bytes)};
+}
 
 LH_DEV float plane_dist(f4 v, int k) {
     switch (k) {
@@ -168,14 +206,45 @@ LH_DEV void raster_small(const Pass& p, SV v0, SV v1, SV v2, int32_t area, int32
     }
 }
 
+// Wave-cooperative raster of one medium triangle (vertex bbox within 64 px,
+// so every edge value fits in int32): the 64 lanes cover its pixel bbox in
+// 8x8 tiles.  All arguments are wave-uniform.
+template <int KIND>
+LH_DEV void raster_wave(const Pass& p, SV v0, SV v1, SV v2, int32_t xlo, int32_t xhi, int32_t ylo, int32_t yhi,
+                        uint32_t id, uint32_t lane) {
+    const int32_t area = (v1.X - v0.X) * (v2.Y - v0.Y) - (v1.Y - v0.Y) * (v2.X - v0.X);
+    const bool tl0 = top_left(v1.X, v1.Y, v2.X, v2.Y), tl1 = top_left(v2.X, v2.Y, v0.X, v0.Y),
+               tl2 = top_left(v0.X, v0.Y, v1.X, v1.Y);
+    const float inv_area = 1.0f / float(area);
+    const float d10 = v1.z - v0.z, d20 = v2.z - v0.z;
+    const int32_t lx = int32_t(lane & 7), ly = int32_t(lane >> 3);
+    for (int32_t ty = ylo; ty <= yhi; ty += 8) {
+        for (int32_t tx = xlo; tx <= xhi; tx += 8) {
+            const int32_t px = tx + lx, py = ty + ly;
+            if (px > xhi || py > yhi) continue;
+            const int32_t Px = px * 256 + 128, Py = py * 256 + 128;
+            const int32_t e0 = (v2.X - v1.X) * (Py - v1.Y) - (v2.Y - v1.Y) * (Px - v1.X);
+            const int32_t e1 = (v0.X - v2.X) * (Py - v2.Y) - (v0.Y - v2.Y) * (Px - v2.X);
+            const int32_t e2 = (v1.X - v0.X) * (Py - v0.Y) - (v1.Y - v0.Y) * (Px - v0.X);
+            if ((e0 > 0 || (e0 == 0 && tl0)) && (e1 > 0 || (e1 == 0 && tl1)) && (e2 > 0 || (e2 == 0 && tl2))) {
+                const float l1 = float(e1) * inv_area, l2 = float(e2) * inv_area;
+                fragment<KIND>(p, px, py, (v0.z + l1 * d10) + l2 * d20, id);
+            }
+        }
+    }
+}
+
+LH_DEV int32_t readlane(int32_t v, uint32_t l) { return __builtin_amdgcn_readlane(v, int(l)); }
+LH_DEV float readlane(float v, uint32_t l) { return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), int(l))); }
+
 LH_DEV void store_big(BigTri* q, const SV& v0, const SV& v1, const SV& v2, uint32_t id, int32_t xlo, int32_t xhi,
-                      int32_t ylo, int32_t yhi) {
+                      int32_t ylo, int32_t yhi, uint32_t pi) {
     BigTri b;
     b.X[0] = v0.X, b.X[1] = v1.X, b.X[2] = v2.X;
     b.Y[0] = v0.Y, b.Y[1] = v1.Y, b.Y[2] = v2.Y;
     b.z[0] = v0.z, b.z[1] = v1.z, b.z[2] = v2.z;
     b.id = id;
-    b.xlo = uint16_t(xlo), b.ylo = uint16_t(ylo), b.xhi = uint16_t(xhi), b.yhi = uint16_t(yhi);
+    b.xlo = uint16_t(uint32_t(xlo) | (pi << 13)), b.ylo = uint16_t(ylo), b.xhi = uint16_t(xhi), b.yhi = uint16_t(yhi);
     uint4* dst = reinterpret_cast<uint4*>(q);
     const uint4* src = reinterpret_cast<const uint4*>(&b);
     dst[0] = src[0];
@@ -203,17 +272,69 @@ LH_DEV const LhMat4& view_matrix(const lh_view& v, uint32_t pass) {
     return KIND == 0 ? v.uniforms->combined_perspective_view : v.misc->shadow_matrices[pass - 1];
 }
 
-template <int KIND>
-__global__ void __launch_bounds__(64) k_raster_meshlets(lh_scene s, lh_view v, Pass p) {
+// Segment lookup of a flattened draw index g.  One view: the segments are the
+// passes of the set, offsets computed from the view's counters.  A batch: the
+// segments are (view, pass) pairs, offsets planned on the device beforehand.
+template <class VS>
+struct Segments;
+
+template <>
+struct Segments<OneView> {
+    uint32_t off[LH_NUM_CASCADES + 1];
+    uint32_t n;
+    LH_DEV Segments(const OneView& vs, const PassSet& ps) : n(ps.n) {
+        const lh_view& v = vs.v;
+        off[0] = 0;
+        for (uint32_t i = 0; i < LH_NUM_CASCADES; i++)
+            off[i + 1] = off[i] + (i < ps.n ? min(v.counters[LH_CNT_DRAWS + ps.pass0 + i], v.draw_capacity) : 0u);
+    }
+    LH_DEV uint32_t total() const { return off[n]; }
+    LH_DEV void find(uint32_t g, uint32_t& view, uint32_t& pi, uint32_t& d) const {
+        pi = 0;
+        while (pi + 1 < n && g >= off[pi + 1]) pi++;
+        view = 0;
+        d = g - off[pi];
+    }
+};
+
+template <>
+struct Segments<ViewTable> {
+    const uint32_t* off;
+    uint32_t nseg, np;
+    LH_DEV Segments(const ViewTable& vs, const PassSet& ps) : off(vs.seg_offsets), nseg(vs.n * ps.n), np(ps.n) {}
+    LH_DEV uint32_t total() const { return off[nseg]; }
+    LH_DEV void find(uint32_t g, uint32_t& view, uint32_t& pi, uint32_t& d) const {
+        uint32_t lo = 0, hi = nseg;  // last k with off[k] <= g
+        while (hi - lo > 1) {
+            uint32_t mid = (lo + hi) / 2;
+            if (off[mid] <= g)
+                lo = mid;
+            else
+                hi = mid;
+        }
+        view = lo / np;
+        pi = lo % np;
+        d = g - off[lo];
+    }
+};
+
+template <int KIND, class VS>
+__global__ void __launch_bounds__(64) LH_RASTER_OCCUPANCY k_raster_meshlets(lh_scene s, VS vs, PassSet ps) {
     __shared__ int32_t sX[64], sY[64];
     __shared__ float sZ[64];
     __shared__ uint32_t sO[64];
     const uint32_t lane = threadIdx.x;
-    const LhMat4 M = view_matrix<KIND>(v, p.pass);
-    const uint32_t count = min(v.counters[LH_CNT_DRAWS + p.pass], v.draw_capacity);
-    uint32_t* clip_counter = &v.counters[LH_CNT_CLIP + p.pass];
-    uint32_t* big_counter = &v.counters[LH_CNT_BIG + p.pass];
-    for (uint32_t d = blockIdx.x; d < count; d += gridDim.x) {
+    const Segments<VS> seg(vs, ps);
+    const uint32_t total = seg.total();
+    for (uint32_t g = blockIdx.x; g < total; g += gridDim.x) {
+        uint32_t view, pi, d;
+        seg.find(g, view, pi, d);
+        const lh_view& v = vs.get(view);
+        const Pass p = make_pass(v, ps.pass0 + pi);
+        const Queues qs = queues(v);
+        uint32_t* clip_counter = &v.counters[LH_CNT_CLIP + ps.pass0];
+        uint32_t* big_counter = &v.counters[LH_CNT_BIG + ps.pass0];
+        const LhMat4 M = view_matrix<KIND>(v, p.pass);
         const MeshletReference ref = p.draws[d];
         const Instance& in = s.instances[ref.instance_index];
         const MeshInfo& mi = mesh_info_at(in.mesh_info_address);
@@ -240,7 +361,8 @@ __global__ void __launch_bounds__(64) k_raster_meshlets(lh_scene s, lh_view v, P
         const uint8_t* micro = ptr<uint8_t>(mi.micro_indices) + ml.triangle_offset;
         for (uint32_t round = 0; round < 2; round++) {
             const uint32_t t = lane + 64 * round;
-            // 0 none, 1 clip queue, 2 big queue, 3 in-lane
+            if (__ballot(t < ntri) == 0) break;
+            // 0 none, 1 clip queue, 2 big queue, 3 in-lane, 4 wave-cooperative
             uint32_t cls = 0;
             SV v0{}, v1{}, v2{};
             int64_t area = 0;
@@ -255,28 +377,38 @@ __global__ void __launch_bounds__(64) k_raster_meshlets(lh_scene s, lh_view v, P
                         v0 = SV{sX[i0], sY[i0], sZ[i0]};
                         v1 = SV{sX[i1], sY[i1], sZ[i1]};
                         v2 = SV{sX[i2], sY[i2], sZ[i2]};
-                        if (setup<KIND>(v0, v1, v2, p.W, p.H, area, xlo, xhi, ylo, yhi, ex, ey))
-                            cls = (ex <= p.small_extent && ey <= p.small_extent) ? 3u : 2u;
+                        if (setup<KIND>(v0, v1, v2, p.W, p.H, area, xlo, xhi, ylo, yhi, ex, ey)) {
+                            const int32_t e = max(ex, ey);
+                            cls = e <= ps.small_extent ? 3u : e <= ps.medium_extent ? 4u : 2u;
+                        }
                     }
                 }
             }
-            if (__ballot(t < ntri) == 0) break;
             const uint32_t id = (mid << LH_KEY_TRI_BITS) | t;
             uint32_t cslot = wave_append(clip_counter, cls == 1);
             if (cls == 1) {
-                if (cslot < p.clip_cap)
-                    p.clip_queue[cslot] = (d << LH_KEY_TRI_BITS) | t;
+                if (cslot < v.clip_capacity)
+                    qs.clip[cslot] = make_uint2((d << LH_KEY_TRI_BITS) | t, pi);
                 else
                     atomicOr(&v.counters[LH_CNT_OVERFLOW], LH_OVERFLOW_CLIP);
             }
             uint32_t bslot = wave_append(big_counter, cls == 2);
             if (cls == 2) {
-                if (bslot < p.big_cap)
-                    store_big(p.big_queue + bslot, v0, v1, v2, id, xlo, xhi, ylo, yhi);
+                if (bslot < v.big_capacity)
+                    store_big(qs.big + bslot, v0, v1, v2, id, xlo, xhi, ylo, yhi, pi);
                 else
                     atomicOr(&v.counters[LH_CNT_OVERFLOW], LH_OVERFLOW_BIG);
             }
             if (cls == 3) raster_small<KIND>(p, v0, v1, v2, int32_t(area), xlo, xhi, ylo, yhi, id);
+            // Medium triangles: broadcast each lane's setup and raster it with the whole wave.
+            for (uint64_t med = ballot(cls == 4); med; med &= med - 1) {
+                const uint32_t l = first_lane(med);
+                const SV a{readlane(v0.X, l), readlane(v0.Y, l), readlane(v0.z, l)};
+                const SV b{readlane(v1.X, l), readlane(v1.Y, l), readlane(v1.z, l)};
+                const SV c{readlane(v2.X, l), readlane(v2.Y, l), readlane(v2.z, l)};
+                raster_wave<KIND>(p, a, b, c, readlane(xlo, l), readlane(xhi, l), readlane(ylo, l), readlane(yhi, l),
+                                  (mid << LH_KEY_TRI_BITS) | (l + 64 * round), lane);
+            }
         }
         __syncthreads();
     }
@@ -288,14 +420,18 @@ LH_DEV f4 clip_point(f4 a, f4 b, float da, float db) {
     return {a.x + (b.x - a.x) * t, a.y + (b.y - a.y) * t, a.z + (b.z - a.z) * t, a.w + (b.w - a.w) * t};
 }
 
-template <int KIND>
-__global__ void __launch_bounds__(64) k_raster_clip(lh_scene s, lh_view v, Pass p) {
-    const LhMat4 M = view_matrix<KIND>(v, p.pass);
-    const uint32_t count = min(v.counters[LH_CNT_CLIP + p.pass], p.clip_cap);
-    uint32_t* big_counter = &v.counters[LH_CNT_BIG + p.pass];
+// View blockIdx.z; grid-stride over that view's clip queue.
+template <int KIND, class VS>
+__global__ void __launch_bounds__(64) k_raster_clip(lh_scene s, VS vs, PassSet ps) {
+    const lh_view& v = vs.get(blockIdx.z);
+    const Queues qs = queues(v);
+    const uint32_t count = min(v.counters[LH_CNT_CLIP + ps.pass0], v.clip_capacity);
+    uint32_t* big_counter = &v.counters[LH_CNT_BIG + ps.pass0];
     for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < count; q += gridDim.x * blockDim.x) {
-        const uint32_t e = p.clip_queue[q];
-        const uint32_t d = e >> LH_KEY_TRI_BITS, t = e & ((1u << LH_KEY_TRI_BITS) - 1);
+        const uint2 e = qs.clip[q];
+        const uint32_t d = e.x >> LH_KEY_TRI_BITS, t = e.x & ((1u << LH_KEY_TRI_BITS) - 1), pi = e.y;
+        const Pass p = make_pass(v, ps.pass0 + pi);
+        const LhMat4 M = view_matrix<KIND>(v, p.pass);
         const MeshletReference ref = p.draws[d];
         const Instance& in = s.instances[ref.instance_index];
         const MeshInfo& mi = mesh_info_at(in.mesh_info_address);
@@ -332,28 +468,31 @@ __global__ void __launch_bounds__(64) k_raster_clip(lh_scene s, lh_view v, Pass 
             int32_t xlo, xhi, ylo, yhi, ex, ey;
             if (!setup<KIND>(a, b, c, p.W, p.H, area, xlo, xhi, ylo, yhi, ex, ey)) continue;
             uint32_t slot = atomicAdd(big_counter, 1u);
-            if (slot < p.big_cap)
-                store_big(p.big_queue + slot, a, b, c, id, xlo, xhi, ylo, yhi);
+            if (slot < v.big_capacity)
+                store_big(qs.big + slot, a, b, c, id, xlo, xhi, ylo, yhi, pi);
             else
                 atomicOr(&v.counters[LH_CNT_OVERFLOW], LH_OVERFLOW_BIG);
         }
     }
 }
 
-// One 256-thread block per queued triangle; the bbox is walked in 16x16 tiles
-// and each pixel evaluates the int64 edge functions directly.
-template <int KIND>
-__global__ void __launch_bounds__(256) k_raster_big(lh_view v, Pass p) {
-    const uint32_t count = min(v.counters[LH_CNT_BIG + p.pass], p.big_cap);
+// View blockIdx.z; one 256-thread block per queued triangle; the bbox is
+// walked in 16x16 tiles and each pixel evaluates the int64 edge functions.
+template <int KIND, class VS>
+__global__ void __launch_bounds__(256) k_raster_big(VS vs, PassSet ps) {
+    const lh_view& v = vs.get(blockIdx.z);
+    const Queues qs = queues(v);
+    const uint32_t count = min(v.counters[LH_CNT_BIG + ps.pass0], v.big_capacity);
     const int32_t tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
     for (uint32_t e = blockIdx.x; e < count; e += gridDim.x) {
-        const BigTri b = p.big_queue[e];
+        const BigTri b = qs.big[e];
+        const Pass p = make_pass(v, ps.pass0 + (uint32_t(b.xlo) >> 13));
         const int32_t X0 = b.X[0], X1 = b.X[1], X2 = b.X[2], Y0 = b.Y[0], Y1 = b.Y[1], Y2 = b.Y[2];
         const int64_t area = edge64(X0, Y0, X1, Y1, X2, Y2);
         const float inv_area = 1.0f / float(area);
         const float z0 = b.z[0], d10 = b.z[1] - b.z[0], d20 = b.z[2] - b.z[0];
         const bool tl0 = top_left(X1, Y1, X2, Y2), tl1 = top_left(X2, Y2, X0, Y0), tl2 = top_left(X0, Y0, X1, Y1);
-        const int32_t xlo = b.xlo, xhi = b.xhi, ylo = b.ylo, yhi = b.yhi;
+        const int32_t xlo = b.xlo & 0x1FFF, xhi = b.xhi, ylo = b.ylo, yhi = b.yhi;
         for (int32_t by = ylo; by <= yhi; by += 16) {
             for (int32_t bx = xlo; bx <= xhi; bx += 16) {
                 const int32_t px = bx + tx, py = by + ty;
@@ -372,37 +511,72 @@ __global__ void __launch_bounds__(256) k_raster_big(lh_view v, Pass p) {
     }
 }
 
+// Batch plan: off[k + 1] = off[k] + surviving meshlets of segment k
+// (segment = view k / np, pass pass0 + k % np), scanned in LDS by one block.
+__global__ void __launch_bounds__(256) k_plan_segments(ViewTable vs, uint32_t pass0, uint32_t np) {
+    __shared__ uint32_t cnt[LH_MAX_BATCH_SEGMENTS];
+    const uint32_t nseg = vs.n * np;
+    for (uint32_t k = threadIdx.x; k < nseg; k += blockDim.x) {
+        const lh_view& v = vs.views[k / np];
+        cnt[k] = min(v.counters[LH_CNT_DRAWS + pass0 + k % np], v.draw_capacity);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t acc = 0;
+        for (uint32_t k = 0; k < nseg; k++) {
+            uint32_t c = cnt[k];
+            cnt[k] = acc;
+            acc += c;
+        }
+        vs.seg_offsets[nseg] = acc;
+    }
+    __syncthreads();
+    for (uint32_t k = threadIdx.x; k < nseg; k += blockDim.x) vs.seg_offsets[k] = cnt[k];
+}
+
 }  // namespace
 
 namespace lhk {
 
-hipError_t raster(const lh_scene& s, const lh_view& v, uint32_t pass, const RasterTuning& t, hipStream_t st) {
-    Pass p{};
-    p.pass = pass;
-    if (pass == 0) {
-        p.W = v.width;
-        p.H = v.height;
-        p.vis = v.visbuffer;
+template <class VS>
+hipError_t raster(const lh_scene& s, const VS& vs, uint32_t pass0, uint32_t npasses, const RasterTuning& t,
+                  hipStream_t st) {
+    if (npasses == 0 || npasses > LH_NUM_CASCADES || pass0 + npasses > LH_NUM_PASSES || (pass0 == 0 && npasses != 1))
+        return hipErrorInvalidValue;
+    const uint32_t small_px = std::min<uint32_t>(std::max<uint32_t>(t.small_extent_px, 1), 32);
+    const uint32_t medium_px = std::min<uint32_t>(std::max(t.medium_extent_px, small_px), 64);
+    PassSet ps{pass0, npasses, int32_t(small_px * 256), int32_t(medium_px * 256)};
+    const uint32_t nv = vs.count();
+    // per-view persistent grids of the queue kernels shrink with the batch size
+    const uint32_t clip_blocks = nv > 1 ? std::max(t.clip_blocks / 16, 4u) : t.clip_blocks;
+    const uint32_t big_blocks = nv > 1 ? std::max(t.big_blocks / 16, 32u) : t.big_blocks;
+    if (pass0 == 0) {
+        mark("raster_meshlets_view", st);
+        hipLaunchKernelGGL((k_raster_meshlets<0, VS>), dim3(t.meshlet_blocks), dim3(64), 0, st, s, vs, ps);
+        mark("raster_clip_view", st);
+        hipLaunchKernelGGL((k_raster_clip<0, VS>), dim3(clip_blocks, 1, nv), dim3(64), 0, st, s, vs, ps);
+        mark("raster_big_view", st);
+        hipLaunchKernelGGL((k_raster_big<0, VS>), dim3(big_blocks, 1, nv), dim3(256), 0, st, vs, ps);
     } else {
-        p.W = p.H = v.shadow_size;
-        p.depth = v.shadow_maps + size_t(pass - 1) * v.shadow_size * v.shadow_size;
+        mark("raster_meshlets_shadow", st);
+        hipLaunchKernelGGL((k_raster_meshlets<1, VS>), dim3(t.meshlet_blocks), dim3(64), 0, st, s, vs, ps);
+        mark("raster_clip_shadow", st);
+        hipLaunchKernelGGL((k_raster_clip<1, VS>), dim3(clip_blocks, 1, nv), dim3(64), 0, st, s, vs, ps);
+        mark("raster_big_shadow", st);
+        hipLaunchKernelGGL((k_raster_big<1, VS>), dim3(big_blocks, 1, nv), dim3(256), 0, st, vs, ps);
     }
-    p.draws = v.draws + size_t(pass) * v.draw_capacity;
-    p.clip_queue = static_cast<uint32_t*>(v.scratch);
-    uint64_t clip_bytes = (uint64_t(v.clip_capacity) * LH_CLIP_ENTRY_BYTES + 255) & ~uint64_t(255);
-    p.big_queue = reinterpret_cast<BigTri*>(static_cast<char*>(v.scratch) + clip_bytes);
-    p.clip_cap = v.clip_capacity;
-    p.big_cap = v.big_capacity;
-    p.small_extent = int32_t(std::min<uint32_t>(std::max<uint32_t>(t.small_extent_px, 1), 32) * 256);
-    if (pass == 0) {
-        hipLaunchKernelGGL(k_raster_meshlets<0>, dim3(t.meshlet_blocks), dim3(64), 0, st, s, v, p);
-        hipLaunchKernelGGL(k_raster_clip<0>, dim3(t.clip_blocks), dim3(64), 0, st, s, v, p);
-        hipLaunchKernelGGL(k_raster_big<0>, dim3(t.big_blocks), dim3(256), 0, st, v, p);
-    } else {
-        hipLaunchKernelGGL(k_raster_meshlets<1>, dim3(t.meshlet_blocks), dim3(64), 0, st, s, v, p);
-        hipLaunchKernelGGL(k_raster_clip<1>, dim3(t.clip_blocks), dim3(64), 0, st, s, v, p);
-        hipLaunchKernelGGL(k_raster_big<1>, dim3(t.big_blocks), dim3(256), 0, st, v, p);
-    }
+    return hipGetLastError();
+}
+
+template hipError_t raster<OneView>(const lh_scene&, const OneView&, uint32_t, uint32_t, const RasterTuning&,
+                                    hipStream_t);
+template hipError_t raster<ViewTable>(const lh_scene&, const ViewTable&, uint32_t, uint32_t, const RasterTuning&,
+                                      hipStream_t);
+
+hipError_t plan_segments(const ViewTable& vt, uint32_t pass0, uint32_t npasses, hipStream_t st) {
+    if (vt.n * npasses > LH_MAX_BATCH_SEGMENTS) return hipErrorInvalidValue;
+    mark("plan_segments", st);
+    hipLaunchKernelGGL(k_plan_segments, dim3(1), dim3(256), 0, st, vt, pass0, npasses);
     return hipGetLastError();
 }
 

@@ -5,7 +5,7 @@
 //               BRDF + Filament terms    src/shaders/common/pbr.glsl:1-96
 //   k_tonemap   display_transform        src/shaders/display_transform.comp:5-81
 //
-// One thread per pixel, 8x8 blocks (the reference's workgroup shape).  The
+// One thread per pixel, 8x8 tiles (the reference's workgroup shape).  The
 // per-pixel dependent chain key -> meshlet table -> instance -> mesh info ->
 // meshlet -> micro indices -> remap -> vertices is the reference's; the key
 // carries a stable meshlet id, resolved through the scene's meshlet table.
@@ -14,14 +14,19 @@
 // reproduce the reference), a constant sky in place of the absent skybox,
 // isotropic trilinear filtering in place of 16x anisotropy, and the absent
 // tony-mc-mapface LUT replaced by its own x/(x+1) pre-encoding.
-#include "lh_device.h"
+//
+// A batch flattens the 8x8 tiles of all its views into one grid (per-view
+// tile offsets planned on the host), so views of very different sizes share
+// one launch without idle blocks.
 #include "kernels.h"
 #include "shading.h"
 
 using namespace lhd;
 
-__global__ void __launch_bounds__(64) k_resolve(lh_scene s, lh_view v) {
-    const uint32_t x = blockIdx.x * 8 + (threadIdx.x & 7), y = blockIdx.y * 8 + (threadIdx.x >> 3);
+namespace {
+
+LH_DEV void resolve_tile(const lh_scene& s, const lh_view& v, uint32_t tx, uint32_t ty) {
+    const uint32_t x = tx * 8 + (threadIdx.x & 7), y = ty * 8 + (threadIdx.x >> 3);
     if (x >= v.width || y >= v.height) return;
     const f3 c = shading::resolve_pixel(s, v, x, y);
     const size_t o = size_t(y) * v.width + x;
@@ -29,7 +34,30 @@ __global__ void __launch_bounds__(64) k_resolve(lh_scene s, lh_view v) {
     if (v.ldr) v.ldr[o] = shading::tonemap_pixel(c);
 }
 
-__global__ void __launch_bounds__(256) k_tonemap(lh_view v) {
+}  // namespace
+
+__global__ void __launch_bounds__(64) LH_RESOLVE_OCCUPANCY k_resolve(lh_scene s, OneView vs) {
+    resolve_tile(s, vs.v, blockIdx.x, blockIdx.y);
+}
+
+__global__ void __launch_bounds__(64) LH_RESOLVE_OCCUPANCY k_resolve_batch(lh_scene s, ViewTable vs) {
+    const uint32_t g = blockIdx.x;
+    uint32_t lo = 0, hi = vs.n;  // last view with tile_offsets[view] <= g
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) / 2;
+        if (vs.tile_offsets[mid] <= g)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    const lh_view& v = vs.views[lo];
+    const uint32_t local = g - vs.tile_offsets[lo], tiles_x = (v.width + 7) / 8;
+    resolve_tile(s, v, local % tiles_x, local / tiles_x);
+}
+
+template <class VS>
+__global__ void __launch_bounds__(256) k_tonemap(VS vs) {
+    const lh_view& v = vs.get(blockIdx.z);
     const uint64_t n = uint64_t(v.width) * v.height;
     for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += uint64_t(gridDim.x) * blockDim.x) {
         float4 c = reinterpret_cast<const float4*>(v.radiance)[i];
@@ -39,17 +67,32 @@ __global__ void __launch_bounds__(256) k_tonemap(lh_view v) {
 
 namespace lhk {
 
-hipError_t resolve(const lh_scene& s, const lh_view& v, hipStream_t st) {
-    dim3 grid((v.width + 7) / 8, (v.height + 7) / 8);
-    hipLaunchKernelGGL(k_resolve, grid, dim3(64), 0, st, s, v);
+template <>
+hipError_t resolve<OneView>(const lh_scene& s, const OneView& vs, uint32_t, uint32_t, hipStream_t st) {
+    dim3 grid((vs.v.width + 7) / 8, (vs.v.height + 7) / 8);
+    mark("resolve", st);
+    hipLaunchKernelGGL(k_resolve, grid, dim3(64), 0, st, s, vs);
     return hipGetLastError();
 }
 
-hipError_t tonemap(const lh_view& v, hipStream_t st) {
-    uint64_t n = uint64_t(v.width) * v.height;
-    uint32_t blocks = uint32_t(std::min<uint64_t>((n + 255) / 256, 4096));
-    hipLaunchKernelGGL(k_tonemap, dim3(std::max(blocks, 1u)), dim3(256), 0, st, v);
+// max_w carries the total tile count of the batch (planned on the host).
+template <>
+hipError_t resolve<ViewTable>(const lh_scene& s, const ViewTable& vs, uint32_t total_tiles, uint32_t,
+                              hipStream_t st) {
+    mark("resolve", st);
+    hipLaunchKernelGGL(k_resolve_batch, dim3(std::max(total_tiles, 1u)), dim3(64), 0, st, s, vs);
     return hipGetLastError();
 }
+
+template <class VS>
+hipError_t tonemap(const VS& vs, uint64_t max_pixels, hipStream_t st) {
+    uint32_t blocks = uint32_t(std::min<uint64_t>((max_pixels + 255) / 256, 4096));
+    mark("tonemap", st);
+    hipLaunchKernelGGL(k_tonemap<VS>, dim3(std::max(blocks, 1u), 1, vs.count()), dim3(256), 0, st, vs);
+    return hipGetLastError();
+}
+
+template hipError_t tonemap<OneView>(const OneView&, uint64_t, hipStream_t);
+template hipError_t tonemap<ViewTable>(const ViewTable&, uint64_t, hipStream_t);
 
 }  // namespace lhk

@@ -100,10 +100,12 @@ def test_host_only_entry_points_without_gpu():
 
     L = lib()
     assert L.lh_version().startswith(b"lighthugger_amd")
-    assert L.lh_scratch_bytes(10, 100) == abi.scratch_bytes(10, 100) == 512 + 480
+    assert L.lh_scratch_bytes(10, 100) == abi.scratch_bytes(10, 100) == 1024 + 480
     t = get_tuning()
     assert 1 <= t.small_extent_px <= 32
-    bad = Tuning(0, 1, 1, 1, 1)
+    bad = Tuning(0, 1, 1, 1, 1, 8)
+    assert L.lh_set_tuning(C.byref(bad)) == abi.LH_ERR_INVALID
+    bad = Tuning(8, 1, 1, 1, 1, 4)  # medium below small
     assert L.lh_set_tuning(C.byref(bad)) == abi.LH_ERR_INVALID
     set_tuning(t)
     # argument validation happens before any HIP call

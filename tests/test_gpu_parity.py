@@ -44,6 +44,138 @@ def test_small_frame_parity():
     assert rep["covered_pixels"] > 10000
 
 
+def test_large_frame_parity_full_size():
+    """The benchmark workload at full size: 2,000 instances, 1920x1080, 4 x 2048^2."""
+    scene = HostScene.preset(abi.PRESET_LARGE)
+    cam = preset_camera(abi.PRESET_LARGE)
+    rep = _run(scene, cam, ViewConfig(cam.width, cam.height, 2048, big_capacity=1 << 21))
+    assert rep["pass0"]["meshlets"] > 10000
+
+
+def test_fixtures_match_oracle_and_golden():
+    import hashlib
+    import json
+    from pathlib import Path
+
+    from .fixtures import fixtures
+
+    golden = json.loads((Path(__file__).resolve().parent / "golden" / "fixtures.json").read_text())
+    for name, (scene, cam, cfg) in fixtures().items():
+        print("fixture", name, flush=True)
+        gpu = _gpu_frame(scene, cam, cfg)
+        orc = oracle_frame(scene, cfg, make_uniforms(cam, scene.num_instances))
+        compare_frames(gpu, orc, scene)
+        assert hashlib.sha256(np.ascontiguousarray(gpu["visbuffer"]).tobytes()).hexdigest() == \
+            golden[name]["visbuffer_sha256"], name
+        assert hashlib.sha256(np.ascontiguousarray(gpu["shadow"]).tobytes()).hexdigest() == \
+            golden[name]["shadow_sha256"], name
+
+
+def test_per_pass_calls_equal_lh_frame():
+    """Calling the passes one by one through the C ABI (the reference's
+    render() order) gives the same keys as lh_frame; lh_frame_profiled too."""
+    import torch
+
+    from lighthugger_amd.lib import Renderer
+
+    scene = HostScene.preset(abi.PRESET_SMALL)
+    cam = preset_camera(abi.PRESET_SMALL)
+    r = Renderer(scene, ViewConfig(cam.width, cam.height, 512))
+    r.set_camera(cam)
+    r.frame()
+    torch.cuda.synchronize()
+    a = snapshot(r.view)
+    r.call("lh_view_reset")
+    r.call("lh_cull_instances")
+    r.call("lh_cull_meshlets", 0)
+    r.call("lh_raster_view")
+    r.call("lh_depth_bounds")
+    r.call("lh_fit_cascades")
+    r.call("lh_cull_instances_shadow")
+    for c in range(4):
+        r.call("lh_cull_meshlets", 1 + c)
+        r.call("lh_raster_shadow", c)
+    r.call("lh_resolve")
+    torch.cuda.synchronize()
+    b = snapshot(r.view)
+    for k in ("visbuffer", "shadow", "radiance", "ldr"):
+        assert np.array_equal(a[k], b[k]), k
+    times = r.frame_profiled()
+    names = [n for n, _ in times]
+    assert names[:3] == ["view_reset", "clear_targets", "cull_instances"] and names[-1] == "resolve"
+    assert all(ms >= 0 for _, ms in times)
+    c = snapshot(r.view)
+    assert np.array_equal(a["visbuffer"], c["visbuffer"])
+
+
+def test_output_independent_of_tuning():
+    """The autotuner's parameters change speed only: keys, shadow depths and
+    radiance are bit-identical across extreme settings."""
+    import torch
+
+    from lighthugger_amd.lib import Renderer, Tuning, get_tuning, set_tuning
+
+    scene = HostScene.preset(abi.PRESET_LARGE, texture_size=64)
+    cam = preset_camera(abi.PRESET_LARGE)
+    r = Renderer(scene, ViewConfig(960, 540, 1024, big_capacity=1 << 20))
+    from lighthugger_amd.scene import make_camera
+
+    r.set_camera(make_camera(cam.position.tolist(), cam.target.tolist(), 960, 540))
+    saved = get_tuning()
+    ref = None
+    try:
+        for t in (Tuning(1, 8192, 2048, 256, 1024, 1), Tuning(2, 4096, 512, 64, 512, 16),
+                  Tuning(8, 16384, 2048, 256, 2048, 64), Tuning(32, 1024, 64, 16, 128, 64)):
+            set_tuning(t)
+            r.frame()
+            torch.cuda.synchronize()
+            s = snapshot(r.view)
+            assert r.status()[abi.CNT_OVERFLOW] == 0
+            if ref is None:
+                ref = s
+            for k in ("visbuffer", "shadow", "radiance", "ldr"):
+                assert np.array_equal(ref[k], s[k]), (t.as_dict(), k)
+    finally:
+        set_tuning(saved)
+
+
+def test_capacity_overflow_reported_on_device():
+    import torch
+
+    from lighthugger_amd.lib import Renderer
+
+    scene = HostScene.preset(abi.PRESET_SMALL)
+    cam = preset_camera(abi.PRESET_SMALL)
+    r = Renderer(scene, ViewConfig(cam.width, cam.height, 256, instance_capacity=8, draw_capacity=100,
+                                   big_capacity=4, clip_capacity=4))
+    r.set_camera(cam)
+    r.frame()
+    torch.cuda.synchronize()
+    st = r.status()
+    assert st[abi.CNT_OVERFLOW] & 1 and st[abi.CNT_OVERFLOW] & 2
+    count, _, vals = r.view.prefix(0)
+    assert count > 8 and len(vals) == 8
+
+
+def test_empty_view_on_device():
+    import torch
+
+    from lighthugger_amd.lib import Renderer
+    from lighthugger_amd.scene import make_camera
+
+    scene = HostScene.preset(abi.PRESET_SMALL, num_instances=4)
+    cam = make_camera((0, 500, 0), (0, 1000, 1), 64, 48)
+    cfg = ViewConfig(64, 48, 64)
+    r = Renderer(scene, cfg)
+    r.set_camera(cam)
+    r.frame()
+    torch.cuda.synchronize()
+    gpu = snapshot(r.view)
+    assert gpu["counters"][abi.CNT_FLAGS] & 1
+    orc = oracle_frame(scene, cfg, make_uniforms(cam, scene.num_instances))
+    compare_frames(gpu, orc, scene)
+
+
 def test_small_frame_twice_identical():
     """Two frames in a row through the same buffers give identical keys."""
     import torch
