@@ -42,20 +42,69 @@ def parse():
     ap.add_argument("--profile-only", action="store_true", help="run a few frames for an external profiler")
     ap.add_argument("--copy-probe", action="store_true",
                     help="run 256 MiB device-to-device copies (HBM-counter calibration for rocprofv3 --pmc)")
+    ap.add_argument("--no-atomic-probe", action="store_true", help="skip the atomicMin ceiling measurement")
     return ap.parse_args()
 
 
+COPY_PROBE_BYTES = 256 << 20
+
+
+def probe_lib():
+    import ctypes as C
+
+    path = ROOT / "lighthugger_amd" / "liblh_probe.so"
+    if not path.exists():
+        raise RuntimeError(f"{path} is missing: build first (__graft_entry__.build())")
+    L = C.CDLL(str(path))
+    for fn in (L.lhp_atomic_min_u64, L.lhp_atomic_min_u32):
+        fn.restype = C.c_int
+        fn.argtypes = [C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_void_p]
+    L.lhp_copy.restype = C.c_int
+    L.lhp_copy.argtypes = [C.c_void_p, C.c_void_p, C.c_uint64, C.c_void_p]
+    return L
+
+
 def copy_probe(steps: int) -> None:
+    """lhp_copy of COPY_PROBE_BYTES: the known-traffic kernel (k_copy) that
+    scales rocprofv3's FETCH_SIZE / WRITE_SIZE to bytes."""
     import torch
 
-    n = 256 << 20
-    a = torch.empty(n, dtype=torch.uint8, device="cuda")
+    L = probe_lib()
+    a = torch.ones(COPY_PROBE_BYTES, dtype=torch.uint8, device="cuda")
     b = torch.empty_like(a)
-    a.fill_(1)
+    stream = torch.cuda.current_stream().cuda_stream
     for _ in range(max(1, steps)):
-        b.copy_(a)
+        if L.lhp_copy(a.data_ptr(), b.data_ptr(), COPY_PROBE_BYTES, stream):
+            raise RuntimeError("lhp_copy failed")
     torch.cuda.synchronize()
-    print(f"# copy probe: {steps} copies of {n} bytes", file=sys.stderr)
+    assert torch.equal(a, b)
+    print(f"# copy probe: {steps} copies of {COPY_PROBE_BYTES} bytes", file=sys.stderr)
+
+
+def atomic_ceiling(width: int, height: int, bits: int) -> dict:
+    """Device-scope atomicMin throughput (G ops/s) on a width x height buffer of
+    `bits`-bit words: lanes of a wave on one 8x8 tile (the wave-cooperative and
+    big-triangle shape) and on scattered pixels (one triangle per lane)."""
+    import torch
+
+    L = probe_lib()
+    fn = L.lhp_atomic_min_u64 if bits == 64 else L.lhp_atomic_min_u32
+    dt = torch.int64 if bits == 64 else torch.int32
+    buf = torch.full((height * width,), -1 if bits == 32 else (1 << 62), dtype=dt, device="cuda")
+    stream = torch.cuda.current_stream().cuda_stream
+    waves, ops = 1 << 16, 64
+    out = {}
+    for name, pattern in (("tile8x8", 0), ("scattered", 1)):
+        if fn(buf.data_ptr(), width, height, waves, ops, pattern, stream):
+            raise RuntimeError("atomic probe launch failed")
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(5):
+            fn(buf.data_ptr(), width, height, waves, ops, pattern, stream)
+        e1.record()
+        torch.cuda.synchronize()
+        out[name] = round(5 * waves * 64 * ops / (e0.elapsed_time(e1) * 1e-3) / 1e9, 2)
+    return out
 
 
 def meshlet_sizes(scene):
@@ -245,6 +294,24 @@ def main():
     covered = int((vis != np.uint64(abi.LH_VISBUFFER_CLEAR)).sum())
     sizes = meshlet_sizes(scene)
 
+    # ---- raster statistics (one extra frame with LH_VIEW_STATISTICS) ----
+    r.view.c.flags = abi.VIEW_STATISTICS
+    r.frame()
+    torch.cuda.synchronize()
+    sc = r.view.counters()
+    r.view.c.flags = 0
+    frags = [int(x) for x in sc[abi.CNT_FRAGMENTS: abi.CNT_FRAGMENTS + abi.NUM_PASSES]]
+    atomics = [int(x) for x in sc[abi.CNT_ATOMICS: abi.CNT_ATOMICS + abi.NUM_PASSES]]
+    raster_atomics = {"fragments_per_pass": frags, "atomics_per_pass": atomics}
+    for kind, lo, hi in (("view", 0, 1), ("shadow", 1, abi.NUM_PASSES)):
+        kt = sum(per_frame.get(f"raster_{k}_{kind}", 0.0) for k in ("meshlets", "clip", "big"))
+        if kt > 0:
+            raster_atomics[f"{kind}_atomics_per_s_G"] = round(sum(atomics[lo:hi]) / (kt * 1e-3) / 1e9, 2)
+    if not args.no_atomic_probe:
+        raster_atomics["ceiling_u64_1080p_G"] = atomic_ceiling(cfg.width - cfg.width % 8, cfg.height - cfg.height % 8,
+                                                               64)
+        raster_atomics["ceiling_u32_shadow_G"] = atomic_ceiling(shadow, shadow, 32)
+
     roofline = None
     if dominant.startswith("raster_meshlets"):
         if dominant.endswith("_view"):
@@ -257,10 +324,13 @@ def main():
                 algo += raster_algorithmic_bytes(scene, r.view.draws(1 + c), cov_c, sizes) - 4 * cov_c
             algo //= abi.NUM_CASCADES
         achieved = algo / (per_launch[dominant] * 1e-3) / 1e9
+        # HBM bytes per launch from the committed rocprofv3 counter runs
+        # (FETCH_SIZE, WRITE_SIZE scaled by the copy kernel's), when present
         traffic = None
         tfile = ROOT / "profiles" / "traffic.json"
         if tfile.exists():
-            traffic = json.loads(tfile.read_text()).get(dominant)
+            entry = json.loads(tfile.read_text()).get(dominant)
+            traffic = entry["bytes"] if entry else None
         roofline = {"kernel": dominant, "bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
                     "algorithmic_bytes": algo, "avg_launch_ms": round(per_launch[dominant], 5)}
@@ -308,6 +378,7 @@ def main():
         },
         "kernel_ms_per_frame": {k: round(v, 4) for k, v in sorted(per_frame.items(), key=lambda kv: -kv[1])},
         "roofline": roofline,
+        "raster_atomics": raster_atomics,
         "cpu_baseline": cpu,
         "batched_views": batched,
     }

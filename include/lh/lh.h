@@ -51,6 +51,8 @@ typedef void* lh_stream; /* hipStream_t */
 #define LH_CNT_CLIP 16     /* + pass: triangles queued for clipping */
 #define LH_CNT_OVERFLOW 24 /* bit mask of LH_OVERFLOW_* */
 #define LH_CNT_FLAGS 25    /* bit mask of LH_FLAG_* */
+#define LH_CNT_FRAGMENTS 32 /* + pass: covered samples (only with LH_VIEW_STATISTICS) */
+#define LH_CNT_ATOMICS 40   /* + pass: depth atomics issued after the early-out test (same) */
 #define LH_NUM_COUNTERS 64
 #define LH_OVERFLOW_INSTANCES 0x1u
 #define LH_OVERFLOW_DRAWS 0x2u
@@ -58,8 +60,12 @@ typedef void* lh_stream; /* hipStream_t */
 #define LH_OVERFLOW_CLIP 0x8u
 #define LH_FLAG_EMPTY_VIEW 0x1u /* no geometry: depth bounds stayed at their reset values */
 
+/* lh_view.flags */
+#define LH_VIEW_STATISTICS 0x1u /* single-view raster also counts LH_CNT_FRAGMENTS / LH_CNT_ATOMICS
+                                   (slower; batched launches ignore it) */
+
 /* Bytes per entry of the two raster queues inside lh_view.scratch:
- * [clip queue: clip_capacity x 4 B][pad to 256][big queue: big_capacity x 48 B]. */
+ * [clip queue: clip_capacity x 8 B][pad to 256][big queue: big_capacity x 48 B]. */
 #define LH_CLIP_ENTRY_BYTES 8u
 #define LH_BIG_ENTRY_BYTES 48u
 
@@ -96,7 +102,7 @@ typedef struct lh_view {
     uint32_t shadow_size;       /* cascade side (reference: 1024, src/main.cpp:385-409) */
     uint32_t big_capacity;
     uint32_t clip_capacity;
-    uint32_t flags;             /* reserved, 0 */
+    uint32_t flags;             /* LH_VIEW_* bits */
 } lh_view;
 
 /* Bytes of scratch needed for the given queue capacities. */

@@ -4,6 +4,7 @@ Outputs (in-tree, git-ignored, shipped with the tree):
   lighthugger_amd/liblh_scene.so  host scene producer (g++, C++20)
   lighthugger_amd/liblh.so        HIP kernels + C ABI + frame recorder (hipcc, gfx950)
   lighthugger_amd/_lh_torch.so    libtorch autograd Function for the resolve (hipcc)
+  lighthugger_amd/liblh_probe.so  device ceilings for the roofline (atomics, plain copy)
 The oracle (test infrastructure) is built by oracle/build.py.
 
 Floating-point flags are part of the contract: both the kernels and the
@@ -90,8 +91,17 @@ def build_hip_lib(jobs: int = 8) -> Path:
     return out
 
 
+def build_probe_lib() -> Path:
+    BUILD.mkdir(exist_ok=True)
+    obj = BUILD / "probe.o"
+    _run([HIPCC, *HIP_FLAGS, "-c", str(CSRC / "probe.hip"), "-o", str(obj)])
+    out = PKG / "liblh_probe.so"
+    _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", str(obj), "-o", str(out)])
+    return out
+
+
 def build_all() -> list[Path]:
-    outs = [build_scene_lib(), build_hip_lib()]
+    outs = [build_scene_lib(), build_hip_lib(), build_probe_lib()]
     torch_src = CSRC / "torch_ext.cpp"
     if torch_src.exists():
         from . import build_torch  # noqa: PLC0415

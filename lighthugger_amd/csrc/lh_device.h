@@ -114,6 +114,10 @@ LH_DEV uint32_t wave_inclusive_scan(uint32_t v) {
     }
     return v;
 }
+LH_DEV uint32_t wave_sum_u32(uint32_t v) {
+    for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
+    return v;
+}
 LH_DEV uint32_t wave_min_u32(uint32_t v) {
     for (int d = 32; d >= 1; d >>= 1) v = min(v, __shfl_xor(v, d, 64));
     return v;

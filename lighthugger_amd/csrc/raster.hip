@@ -160,27 +160,57 @@ LH_DEV bool setup(SV& v0, SV& v1, SV& v2, uint32_t W, uint32_t H, int64_t& area,
     return xlo <= xhi && ylo <= yhi;
 }
 
+// Raster statistics (LH_VIEW_STATISTICS): covered samples and issued depth
+// atomics per lane, wave-reduced into the view's counters.  Tally<false>
+// compiles to nothing.
+template <bool ON>
+struct Tally {
+    LH_DEV void hit(bool) {}
+    LH_DEV void flush(const lh_view&, uint32_t) {}
+};
+template <>
+struct Tally<true> {
+    uint32_t frags = 0, atomics = 0;
+    LH_DEV void hit(bool issued) {
+        frags++;
+        atomics += issued ? 1u : 0u;
+    }
+    // Called by the whole wave.
+    LH_DEV void flush(const lh_view& v, uint32_t pass) {
+        const uint32_t f = wave_sum_u32(frags), a = wave_sum_u32(atomics);
+        if (lane_id() == 0 && f != 0) {
+            atomicAdd(&v.counters[LH_CNT_FRAGMENTS + pass], f);
+            atomicAdd(&v.counters[LH_CNT_ATOMICS + pass], a);
+        }
+        frags = atomics = 0;
+    }
+};
+
+// Depth test of one covered sample; returns whether an atomic was issued.
 template <int KIND>
-LH_DEV void fragment(const Pass& p, int32_t px, int32_t py, float z, uint32_t id) {
+LH_DEV bool fragment(const Pass& p, int32_t px, int32_t py, float z, uint32_t id) {
     z = fminf(fmaxf(z, 0.0f), 1.0f);
     size_t o = size_t(py) * p.W + uint32_t(px);
     if (KIND == 0) {
-        if (!(z > 0.0f)) return;  // depth GREATER against the 0.0 clear
+        if (!(z > 0.0f)) return false;  // depth GREATER against the 0.0 clear
         unsigned long long key = ((unsigned long long)(~__float_as_uint(z)) << 32) | id;
         unsigned long long* a = reinterpret_cast<unsigned long long*>(p.vis + o);
-        if (key < *a) atomicMin(a, key);  // a stale value is never smaller: safe early-out
+        if (!(key < *a)) return false;  // a stale value is never smaller: safe early-out
+        atomicMin(a, key);
     } else {
         uint32_t d = __float_as_uint(z);
-        if (d < p.depth[o]) atomicMin(p.depth + o, d);
+        if (!(d < p.depth[o])) return false;
+        atomicMin(p.depth + o, d);
     }
+    return true;
 }
 
 // In-lane raster of a triangle whose vertex bbox is within small_extent
 // (<= 32 px): every edge value fits in int32 and the edge functions are
 // stepped exactly in integers.
-template <int KIND>
+template <int KIND, bool ST>
 LH_DEV void raster_small(const Pass& p, SV v0, SV v1, SV v2, int32_t area, int32_t xlo, int32_t xhi, int32_t ylo,
-                         int32_t yhi, uint32_t id) {
+                         int32_t yhi, uint32_t id, Tally<ST>& tally) {
     const bool tl0 = top_left(v1.X, v1.Y, v2.X, v2.Y), tl1 = top_left(v2.X, v2.Y, v0.X, v0.Y),
                tl2 = top_left(v0.X, v0.Y, v1.X, v1.Y);
     const float inv_area = 1.0f / float(area);
@@ -197,7 +227,7 @@ LH_DEV void raster_small(const Pass& p, SV v0, SV v1, SV v2, int32_t area, int32
             bool in = (e0 > 0 || (e0 == 0 && tl0)) && (e1 > 0 || (e1 == 0 && tl1)) && (e2 > 0 || (e2 == 0 && tl2));
             if (in) {
                 float l1 = float(e1) * inv_area, l2 = float(e2) * inv_area;
-                fragment<KIND>(p, px, py, (v0.z + l1 * d10) + l2 * d20, id);
+                tally.hit(fragment<KIND>(p, px, py, (v0.z + l1 * d10) + l2 * d20, id));
             }
             e0 += sx0;
             e1 += sx1;
@@ -209,9 +239,9 @@ LH_DEV void raster_small(const Pass& p, SV v0, SV v1, SV v2, int32_t area, int32
 // Wave-cooperative raster of one medium triangle (vertex bbox within 64 px,
 // so every edge value fits in int32): the 64 lanes cover its pixel bbox in
 // 8x8 tiles.  All arguments are wave-uniform.
-template <int KIND>
+template <int KIND, bool ST>
 LH_DEV void raster_wave(const Pass& p, SV v0, SV v1, SV v2, int32_t xlo, int32_t xhi, int32_t ylo, int32_t yhi,
-                        uint32_t id, uint32_t lane) {
+                        uint32_t id, uint32_t lane, Tally<ST>& tally) {
     const int32_t area = (v1.X - v0.X) * (v2.Y - v0.Y) - (v1.Y - v0.Y) * (v2.X - v0.X);
     const bool tl0 = top_left(v1.X, v1.Y, v2.X, v2.Y), tl1 = top_left(v2.X, v2.Y, v0.X, v0.Y),
                tl2 = top_left(v0.X, v0.Y, v1.X, v1.Y);
@@ -228,7 +258,7 @@ LH_DEV void raster_wave(const Pass& p, SV v0, SV v1, SV v2, int32_t xlo, int32_t
             const int32_t e2 = (v1.X - v0.X) * (Py - v0.Y) - (v1.Y - v0.Y) * (Px - v0.X);
             if ((e0 > 0 || (e0 == 0 && tl0)) && (e1 > 0 || (e1 == 0 && tl1)) && (e2 > 0 || (e2 == 0 && tl2))) {
                 const float l1 = float(e1) * inv_area, l2 = float(e2) * inv_area;
-                fragment<KIND>(p, px, py, (v0.z + l1 * d10) + l2 * d20, id);
+                tally.hit(fragment<KIND>(p, px, py, (v0.z + l1 * d10) + l2 * d20, id));
             }
         }
     }
@@ -280,20 +310,29 @@ struct Segments;
 
 template <>
 struct Segments<OneView> {
+    // off[i]: first flattened index of pass i; only constant indices, so the
+    // array lives in registers.
     uint32_t off[LH_NUM_CASCADES + 1];
-    uint32_t n;
-    LH_DEV Segments(const OneView& vs, const PassSet& ps) : n(ps.n) {
+    LH_DEV Segments(const OneView& vs, const PassSet& ps) {
         const lh_view& v = vs.v;
         off[0] = 0;
+#pragma unroll
         for (uint32_t i = 0; i < LH_NUM_CASCADES; i++)
             off[i + 1] = off[i] + (i < ps.n ? min(v.counters[LH_CNT_DRAWS + ps.pass0 + i], v.draw_capacity) : 0u);
     }
-    LH_DEV uint32_t total() const { return off[n]; }
+    LH_DEV uint32_t total() const { return off[LH_NUM_CASCADES]; }
     LH_DEV void find(uint32_t g, uint32_t& view, uint32_t& pi, uint32_t& d) const {
+        // the last pass whose range starts at or before g (empty passes are skipped)
+        uint32_t base = 0;
         pi = 0;
-        while (pi + 1 < n && g >= off[pi + 1]) pi++;
+#pragma unroll
+        for (uint32_t i = 1; i < LH_NUM_CASCADES; i++)
+            if (g >= off[i]) {
+                pi = i;
+                base = off[i];
+            }
         view = 0;
-        d = g - off[pi];
+        d = g - base;
     }
 };
 
@@ -318,7 +357,7 @@ struct Segments<ViewTable> {
     }
 };
 
-template <int KIND, class VS>
+template <int KIND, class VS, bool ST>
 __global__ void __launch_bounds__(64) LH_RASTER_OCCUPANCY k_raster_meshlets(lh_scene s, VS vs, PassSet ps) {
     __shared__ int32_t sX[64], sY[64];
     __shared__ float sZ[64];
@@ -326,6 +365,7 @@ __global__ void __launch_bounds__(64) LH_RASTER_OCCUPANCY k_raster_meshlets(lh_s
     const uint32_t lane = threadIdx.x;
     const Segments<VS> seg(vs, ps);
     const uint32_t total = seg.total();
+    Tally<ST> tally;
     for (uint32_t g = blockIdx.x; g < total; g += gridDim.x) {
         uint32_t view, pi, d;
         seg.find(g, view, pi, d);
@@ -399,7 +439,7 @@ __global__ void __launch_bounds__(64) LH_RASTER_OCCUPANCY k_raster_meshlets(lh_s
                 else
                     atomicOr(&v.counters[LH_CNT_OVERFLOW], LH_OVERFLOW_BIG);
             }
-            if (cls == 3) raster_small<KIND>(p, v0, v1, v2, int32_t(area), xlo, xhi, ylo, yhi, id);
+            if (cls == 3) raster_small<KIND>(p, v0, v1, v2, int32_t(area), xlo, xhi, ylo, yhi, id, tally);
             // Medium triangles: broadcast each lane's setup and raster it with the whole wave.
             for (uint64_t med = ballot(cls == 4); med; med &= med - 1) {
                 const uint32_t l = first_lane(med);
@@ -407,9 +447,10 @@ __global__ void __launch_bounds__(64) LH_RASTER_OCCUPANCY k_raster_meshlets(lh_s
                 const SV b{readlane(v1.X, l), readlane(v1.Y, l), readlane(v1.z, l)};
                 const SV c{readlane(v2.X, l), readlane(v2.Y, l), readlane(v2.z, l)};
                 raster_wave<KIND>(p, a, b, c, readlane(xlo, l), readlane(xhi, l), readlane(ylo, l), readlane(yhi, l),
-                                  (mid << LH_KEY_TRI_BITS) | (l + 64 * round), lane);
+                                  (mid << LH_KEY_TRI_BITS) | (l + 64 * round), lane, tally);
             }
         }
+        tally.flush(v, p.pass);
         __syncthreads();
     }
 }
@@ -478,12 +519,13 @@ __global__ void __launch_bounds__(64) k_raster_clip(lh_scene s, VS vs, PassSet p
 
 // View blockIdx.z; one 256-thread block per queued triangle; the bbox is
 // walked in 16x16 tiles and each pixel evaluates the int64 edge functions.
-template <int KIND, class VS>
+template <int KIND, class VS, bool ST>
 __global__ void __launch_bounds__(256) k_raster_big(VS vs, PassSet ps) {
     const lh_view& v = vs.get(blockIdx.z);
     const Queues qs = queues(v);
     const uint32_t count = min(v.counters[LH_CNT_BIG + ps.pass0], v.big_capacity);
     const int32_t tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
+    Tally<ST> tally;
     for (uint32_t e = blockIdx.x; e < count; e += gridDim.x) {
         const BigTri b = qs.big[e];
         const Pass p = make_pass(v, ps.pass0 + (uint32_t(b.xlo) >> 13));
@@ -505,9 +547,10 @@ __global__ void __launch_bounds__(256) k_raster_big(VS vs, PassSet ps) {
                 if (!(e1 > 0 || (e1 == 0 && tl1))) continue;
                 if (!(e2 > 0 || (e2 == 0 && tl2))) continue;
                 const float l1 = float(e1) * inv_area, l2 = float(e2) * inv_area;
-                fragment<KIND>(p, px, py, (z0 + l1 * d10) + l2 * d20, b.id);
+                tally.hit(fragment<KIND>(p, px, py, (z0 + l1 * d10) + l2 * d20, b.id));
             }
         }
+        tally.flush(v, p.pass);
     }
 }
 
@@ -538,6 +581,33 @@ __global__ void __launch_bounds__(256) k_plan_segments(ViewTable vs, uint32_t pa
 
 namespace lhk {
 
+template <int KIND, bool ST, class VS>
+void launch_kernels(const lh_scene& s, const VS& vs, const PassSet& ps, uint32_t meshlet_blocks, uint32_t clip_blocks,
+                    uint32_t big_blocks, hipStream_t st) {
+    const uint32_t nv = vs.count();
+    mark(KIND == 0 ? "raster_meshlets_view" : "raster_meshlets_shadow", st);
+    hipLaunchKernelGGL((k_raster_meshlets<KIND, VS, ST>), dim3(meshlet_blocks), dim3(64), 0, st, s, vs, ps);
+    mark(KIND == 0 ? "raster_clip_view" : "raster_clip_shadow", st);
+    hipLaunchKernelGGL((k_raster_clip<KIND, VS>), dim3(clip_blocks, 1, nv), dim3(64), 0, st, s, vs, ps);
+    mark(KIND == 0 ? "raster_big_view" : "raster_big_shadow", st);
+    hipLaunchKernelGGL((k_raster_big<KIND, VS, ST>), dim3(big_blocks, 1, nv), dim3(256), 0, st, vs, ps);
+}
+
+// Statistics are a single-view feature (LH_VIEW_STATISTICS).
+template <int KIND>
+void launch_set(const lh_scene& s, const OneView& vs, const PassSet& ps, uint32_t mb, uint32_t cb, uint32_t bb,
+                hipStream_t st) {
+    if (vs.v.flags & LH_VIEW_STATISTICS)
+        launch_kernels<KIND, true>(s, vs, ps, mb, cb, bb, st);
+    else
+        launch_kernels<KIND, false>(s, vs, ps, mb, cb, bb, st);
+}
+template <int KIND>
+void launch_set(const lh_scene& s, const ViewTable& vs, const PassSet& ps, uint32_t mb, uint32_t cb, uint32_t bb,
+                hipStream_t st) {
+    launch_kernels<KIND, false>(s, vs, ps, mb, cb, bb, st);
+}
+
 template <class VS>
 hipError_t raster(const lh_scene& s, const VS& vs, uint32_t pass0, uint32_t npasses, const RasterTuning& t,
                   hipStream_t st) {
@@ -550,21 +620,10 @@ hipError_t raster(const lh_scene& s, const VS& vs, uint32_t pass0, uint32_t npas
     // per-view persistent grids of the queue kernels shrink with the batch size
     const uint32_t clip_blocks = nv > 1 ? std::max(t.clip_blocks / 16, 4u) : t.clip_blocks;
     const uint32_t big_blocks = nv > 1 ? std::max(t.big_blocks / 16, 32u) : t.big_blocks;
-    if (pass0 == 0) {
-        mark("raster_meshlets_view", st);
-        hipLaunchKernelGGL((k_raster_meshlets<0, VS>), dim3(t.meshlet_blocks), dim3(64), 0, st, s, vs, ps);
-        mark("raster_clip_view", st);
-        hipLaunchKernelGGL((k_raster_clip<0, VS>), dim3(clip_blocks, 1, nv), dim3(64), 0, st, s, vs, ps);
-        mark("raster_big_view", st);
-        hipLaunchKernelGGL((k_raster_big<0, VS>), dim3(big_blocks, 1, nv), dim3(256), 0, st, vs, ps);
-    } else {
-        mark("raster_meshlets_shadow", st);
-        hipLaunchKernelGGL((k_raster_meshlets<1, VS>), dim3(t.meshlet_blocks), dim3(64), 0, st, s, vs, ps);
-        mark("raster_clip_shadow", st);
-        hipLaunchKernelGGL((k_raster_clip<1, VS>), dim3(clip_blocks, 1, nv), dim3(64), 0, st, s, vs, ps);
-        mark("raster_big_shadow", st);
-        hipLaunchKernelGGL((k_raster_big<1, VS>), dim3(big_blocks, 1, nv), dim3(256), 0, st, vs, ps);
-    }
+    if (pass0 == 0)
+        launch_set<0>(s, vs, ps, t.meshlet_blocks, clip_blocks, big_blocks, st);
+    else
+        launch_set<1>(s, vs, ps, t.meshlet_blocks, clip_blocks, big_blocks, st);
     return hipGetLastError();
 }
 

@@ -125,6 +125,7 @@ class ViewConfig:
     big_capacity: int = 1 << 20
     clip_capacity: int = 1 << 18
     ldr: bool = True
+    statistics: bool = False  # LH_VIEW_STATISTICS: count covered samples and depth atomics
 
 
 class ViewData:
@@ -152,7 +153,7 @@ class ViewData:
         self.c = abi.LhView(
             a["uniforms"], a["misc"], a["prefix"], a["draws"], a["counters"], a["visbuffer"], a["shadow"],
             a["radiance"], a["ldr"] if cfg.ldr else 0, a["scratch"], self.sizes["scratch"], icap, dcap, W, H, S,
-            cfg.big_capacity, cfg.clip_capacity, 0,
+            cfg.big_capacity, cfg.clip_capacity, abi.VIEW_STATISTICS if cfg.statistics else 0,
         )
         self.uniforms = None
 

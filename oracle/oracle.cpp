@@ -414,6 +414,7 @@ void raster_pass(const lh_scene& sc, const lh_view& v, uint32_t pass) {
                          v.shadow_maps + size_t(c) * v.shadow_size * v.shadow_size};
         m = &v.misc->shadow_matrices[c];
     }
+    uint64_t frags = 0;
     for (uint32_t d = 0; d < count; d++) {
         const MeshletReference& ref = draws[d];
         const Instance& in = sc.instances[ref.instance_index];
@@ -428,9 +429,12 @@ void raster_pass(const lh_scene& sc, const lh_view& v, uint32_t pass) {
                 F4 q = mat4_mul_vec4(*m, F4{w.x, w.y, w.z, 1.0f});
                 c[j] = CV{q.x, q.y, q.z, q.w};
             }
-            raster_clip_triangle(c, t, (mid << LH_KEY_TRI_BITS) | k);
+            frags += raster_clip_triangle(c, t, (mid << LH_KEY_TRI_BITS) | k);
         }
     }
+    // LH_VIEW_STATISTICS: covered samples of the pass (depth-test order
+    // dependent atomics counts have no sequential counterpart).
+    if (v.flags & LH_VIEW_STATISTICS) v.counters[LH_CNT_FRAGMENTS + pass] += uint32_t(frags);
 }
 
 // ------------------------------------------------------------- resolve

@@ -195,3 +195,27 @@ def test_small_frame_twice_identical():
     assert np.array_equal(a["visbuffer"], b["visbuffer"])
     assert np.array_equal(a["shadow"], b["shadow"])
     assert np.array_equal(a["radiance"], b["radiance"])
+
+
+def test_raster_statistics_match_oracle():
+    """LH_VIEW_STATISTICS: covered samples per pass equal the oracle's count
+    exactly; issued depth atomics are at most the covered samples; the keys are
+    those of a frame without statistics."""
+    scene = HostScene.preset(abi.PRESET_SMALL)
+    cam = preset_camera(abi.PRESET_SMALL)
+    base = ViewConfig(cam.width, cam.height, 512)
+    cfg = ViewConfig(cam.width, cam.height, 512, statistics=True)
+    plain = _gpu_frame(scene, cam, base)
+    gpu = _gpu_frame(scene, cam, cfg)
+    orc = oracle_frame(scene, cfg, make_uniforms(cam, scene.num_instances))
+    compare_frames(gpu, orc, scene)
+    for k in ("visbuffer", "shadow", "radiance"):
+        assert np.array_equal(plain[k], gpu[k]), k
+    g, o = gpu["counters"], orc["counters"]
+    for p in range(abi.NUM_PASSES):
+        frags = int(g[abi.CNT_FRAGMENTS + p])
+        print(f"pass {p}: fragments {frags} atomics {int(g[abi.CNT_ATOMICS + p])}", flush=True)
+        assert frags == int(o[abi.CNT_FRAGMENTS + p]), p
+        assert int(g[abi.CNT_ATOMICS + p]) <= frags and (frags == 0) == (g[abi.CNT_ATOMICS + p] == 0), p
+    assert g[abi.CNT_FRAGMENTS] > 10000
+    assert not plain["counters"][abi.CNT_FRAGMENTS:abi.CNT_FRAGMENTS + abi.NUM_PASSES].any()
