@@ -39,6 +39,7 @@ def parse():
     ap.add_argument("--workload", choices=["large", "small", "stress"], default="large")
     ap.add_argument("--cpu-frames", type=int, default=4, help="oracle frames timed for the CPU baseline (0 = skip)")
     ap.add_argument("--batch-steps", type=int, default=10, help="batched-views timing iterations (0 = skip)")
+    ap.add_argument("--diff-steps", type=int, default=20, help="differentiable-resolve timing iterations (0 = skip)")
     ap.add_argument("--profile-only", action="store_true", help="run a few frames for an external profiler")
     ap.add_argument("--copy-probe", action="store_true",
                     help="run 256 MiB device-to-device copies (HBM-counter calibration for rocprofv3 --pmc)")
@@ -189,6 +190,61 @@ def batch_bench(scene, steps: int, n_cameras: int = 64):
         "speedup_vs_loop": round(rates["batched_graph"]["views_per_s"] / rates["loop"]["views_per_s"], 3),
         "speedup_vs_loop_graph": round(rates["batched_graph"]["views_per_s"] / rates["loop_graph"]["views_per_s"], 3),
     }
+
+
+def diff_bench(steps: int, tri_scale_pct: int = 18, side: int = 1024) -> dict:
+    """Differentiable resolve: the large scene at tri_scale_pct% of its
+    triangles (about 1M visible-candidate triangles), one square view, 4
+    cascades of the same side.  Times the parameterised forward resolve
+    (lh_resolve_ex) and the adjoint (lh_resolve_backward, gradients of every
+    parameter) with HIP events on the stream they run on."""
+    import torch
+
+    from lighthugger_amd import abi
+    from lighthugger_amd.diff import DifferentiableResolve
+    from lighthugger_amd.frame import ViewConfig
+    from lighthugger_amd.lib import Renderer
+    from lighthugger_amd.scene import HostScene, make_camera, preset_camera
+
+    scene = HostScene.preset(abi.PRESET_LARGE, tri_scale_pct=tri_scale_pct)
+    base = preset_camera(abi.PRESET_LARGE)
+    r = Renderer(scene, ViewConfig(side, side, side, big_capacity=1 << 20))
+    r.set_camera(make_camera(base.position.tolist(), base.target.tolist(), side, side))
+    r.frame()
+    torch.cuda.synchronize()
+    _, _, vis0 = r.view.prefix(0)
+    candidates = int(scene.instance_triangles()[vis0["index"].astype(np.int64)].sum())
+    covered = int((r.view.visbuffer() != np.uint64(abi.LH_VISBUFFER_CLEAR)).sum())
+    d = DifferentiableResolve(r)
+    p = d.parameters()
+    g = torch.randn(side, side, 4, device="cuda")
+    out = d.raw_backward(p, g)
+
+    def timed(fn) -> float:
+        fn()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(steps):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / steps
+
+    def backward():
+        for t in out.values():
+            t.zero_()
+        d.raw_backward(p, g, out)
+
+    fwd = timed(lambda: d.forward_raw(p))
+    bwd = timed(backward)
+    zero = timed(lambda: [t.zero_() for t in out.values()])
+    unsupported = d.unsupported()
+    return {"workload": f"large scene at {tri_scale_pct}% triangles, {side}x{side} + 4x{side}^2, visibility fixed",
+            "scene_triangles": int(scene.total_triangles), "visible_candidate_triangles": candidates,
+            "covered_pixels": covered, "forward_ms": round(fwd, 4), "backward_ms": round(bwd - zero, 4),
+            "gradient_zeroing_ms": round(zero, 4), "backward_over_forward": round((bwd - zero) / fwd, 3),
+            "unsupported_pixels": unsupported}
 
 
 def _tuning_of(cls: int) -> dict:
@@ -344,6 +400,11 @@ def main():
         torch.cuda.empty_cache()
         batched = batch_bench(scene, args.batch_steps)
 
+    differentiable = None
+    if args.diff_steps > 0 and args.workload == "large":
+        torch.cuda.empty_cache()
+        differentiable = diff_bench(args.diff_steps)
+
     cpu = None
     if args.cpu_frames > 0:
         cpu = cpu_baseline(scene, cfg, make_uniforms(cam, scene.num_instances), args.cpu_frames)
@@ -381,6 +442,7 @@ def main():
         "raster_atomics": raster_atomics,
         "cpu_baseline": cpu,
         "batched_views": batched,
+        "differentiable_resolve": differentiable,
     }
     print(json.dumps(out), flush=True)
 

@@ -152,6 +152,45 @@ lh_status lh_resolve(const lh_scene* scene, const lh_view* view, lh_stream strea
 /* display_transform as a separate pass: radiance -> BGRA8 ldr. */
 lh_status lh_tonemap(const lh_view* view, lh_stream stream);
 
+/* ---- differentiable resolve (no counterpart in the reference) ----
+ * The resolve as a function of explicit parameters, with the visibility
+ * buffer (the triangle seen by each pixel) held fixed.  Each non-NULL pointer
+ * of lh_resolve_params replaces the value the scene / uniforms hold; with all
+ * pointers NULL lh_resolve_ex is lh_resolve, bit for bit. */
+typedef struct lh_resolve_params {
+    const float* base_color_factor; /* [num_mesh_infos * 3]  (MeshInfo.base_color_factor) */
+    const float* roughness_factor;  /* [num_mesh_infos]      (LhMaterialFactors) */
+    const float* metallic_factor;   /* [num_mesh_infos] */
+    const float* sun_dir;           /* [3]                   (Uniforms.sun_dir, used as given) */
+    const float* sun_intensity;     /* [3]                   (Uniforms.sun_intensity) */
+} lh_resolve_params;
+lh_status lh_resolve_ex(const lh_scene* scene, const lh_view* view, const lh_resolve_params* params,
+                        lh_stream stream);
+
+/* Gradients of sum(grad_radiance * radiance) (radiance RGB of lh_resolve_ex;
+ * alpha ignored), ADDED into the non-NULL outputs.  Vertex positions and
+ * normals: the gradient of the FP32 component stored at device address A of
+ * the geometry arena [geometry_base, geometry_base + geometry_bytes) is added
+ * at geometry[(A - geometry_base) / 4], so the gradient arena mirrors the
+ * geometry arena (meshes shared by several mesh infos / instances accumulate).
+ * Held fixed, like visibility: the back-face flip, the cascade choice and PCF
+ * shadow factor, and the texture LOD (mip levels and their blend weight).
+ * Mesh infos with 16-bit vertices, a normal texture or alpha clip are skipped
+ * and flagged with LH_FLAG_BACKWARD_UNSUPPORTED in the view's counters. */
+typedef struct lh_resolve_grads {
+    float* base_color_factor; /* [num_mesh_infos * 3] */
+    float* roughness_factor;  /* [num_mesh_infos] */
+    float* metallic_factor;   /* [num_mesh_infos] */
+    float* sun_dir;           /* [3] */
+    float* sun_intensity;     /* [3] */
+    float* geometry;          /* [geometry_bytes / 4] */
+    uint64_t geometry_base;
+    uint64_t geometry_bytes;
+} lh_resolve_grads;
+#define LH_FLAG_BACKWARD_UNSUPPORTED 0x2u
+lh_status lh_resolve_backward(const lh_scene* scene, const lh_view* view, const lh_resolve_params* params,
+                              const float* grad_radiance, const lh_resolve_grads* grads, lh_stream stream);
+
 /* The whole frame in the reference's pass order (src/rendering.cpp:79-502). */
 lh_status lh_frame(const lh_scene* scene, const lh_view* view, lh_stream stream);
 

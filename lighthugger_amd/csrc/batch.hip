@@ -54,7 +54,7 @@ hipError_t record_batched(const lh_batch& b, hipStream_t st) {
     if (e == hipSuccess) e = cull_meshlets(s, vt, 1, LH_NUM_CASCADES, g_tunings[kBatchTuningClass].cull_blocks, st);
     if (e == hipSuccess) e = plan_segments(vt, 1, LH_NUM_CASCADES, st);
     if (e == hipSuccess) e = raster(s, vt, 1, LH_NUM_CASCADES, g_tunings[kBatchTuningClass].raster, st);
-    if (e == hipSuccess) e = resolve(s, vt, b.total_tiles, 0, st);
+    if (e == hipSuccess) e = resolve(s, vt, b.total_tiles, lh_resolve_params{}, st);
     return e;
 }
 

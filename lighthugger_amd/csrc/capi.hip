@@ -39,7 +39,7 @@ hipError_t record_frame(const lh_scene& s, const lh_view& v, hipStream_t st) {
     if (e == hipSuccess) e = fit_cull_instances_shadow(s, vs, st);
     if (e == hipSuccess) e = cull_meshlets(s, vs, 1, LH_NUM_CASCADES, tuning_for(v).cull_blocks, st);
     if (e == hipSuccess) e = raster(s, vs, 1, LH_NUM_CASCADES, tuning_for(v).raster, st);
-    if (e == hipSuccess) e = resolve(s, vs, 0, 0, st);
+    if (e == hipSuccess) e = resolve(s, vs, 0, lh_resolve_params{}, st);
     return e;
 }
 
@@ -103,11 +103,29 @@ lh_status lh_raster_shadow(const lh_scene* s, const lh_view* v, uint32_t cascade
     return lhc::hip_status(lhk::raster(*s, one(v), 1 + cascade, 1, lhk::tuning_for(*v).raster, lhc::stream_of(stream)));
 }
 
+static bool valid_resolve(const lh_scene* s, const lh_view* v) {
+    return lhc::valid_scene(s) && lhc::valid_view(v) && v->visbuffer && v->radiance && v->shadow_maps &&
+           v->shadow_size != 0;
+}
+
 lh_status lh_resolve(const lh_scene* s, const lh_view* v, lh_stream stream) {
-    if (!lhc::valid_scene(s) || !lhc::valid_view(v) || !v->visbuffer || !v->radiance || !v->shadow_maps ||
-        v->shadow_size == 0)
-        return LH_ERR_INVALID;
-    return lhc::hip_status(lhk::resolve(*s, one(v), 0, 0, lhc::stream_of(stream)));
+    if (!valid_resolve(s, v)) return LH_ERR_INVALID;
+    return lhc::hip_status(lhk::resolve(*s, one(v), 0, lh_resolve_params{}, lhc::stream_of(stream)));
+}
+
+lh_status lh_resolve_ex(const lh_scene* s, const lh_view* v, const lh_resolve_params* params, lh_stream stream) {
+    if (!valid_resolve(s, v)) return LH_ERR_INVALID;
+    const lh_resolve_params p = params ? *params : lh_resolve_params{};
+    return lhc::hip_status(lhk::resolve(*s, one(v), 0, p, lhc::stream_of(stream)));
+}
+
+lh_status lh_resolve_backward(const lh_scene* s, const lh_view* v, const lh_resolve_params* params,
+                              const float* grad_radiance, const lh_resolve_grads* grads, lh_stream stream) {
+    if (!valid_resolve(s, v) || !grad_radiance || !grads) return LH_ERR_INVALID;
+    if (grads->geometry && (grads->geometry_bytes == 0 || grads->geometry_bytes % 4)) return LH_ERR_INVALID;
+    if (!v->scratch || v->scratch_bytes < lhk::kBackwardScratchBytes) return LH_ERR_CAPACITY;
+    const lh_resolve_params p = params ? *params : lh_resolve_params{};
+    return lhc::hip_status(lhk::resolve_backward(*s, one(v), p, grad_radiance, *grads, lhc::stream_of(stream)));
 }
 
 lh_status lh_tonemap(const lh_view* v, lh_stream stream) {

@@ -67,8 +67,17 @@ template <class VS>
 hipError_t depth_bounds(const VS& vs, uint64_t max_pixels, hipStream_t st);
 template <class VS>
 hipError_t fit_cascades(const VS& vs, hipStream_t st);
+// total_tiles: 8x8 tiles over all views of a batch (ignored for one view);
+// params: the differentiable-parameter overrides (all NULL = scene values).
 template <class VS>
-hipError_t resolve(const lh_scene& s, const VS& vs, uint32_t max_w, uint32_t max_h, hipStream_t st);
+hipError_t resolve(const lh_scene& s, const VS& vs, uint32_t total_tiles, const lh_resolve_params& params,
+                   hipStream_t st);
+// Adjoint of the single-view resolve (lh_resolve_backward); uses the first
+// kBackwardScratchBytes of the view's scratch for per-block partial sums.
+constexpr uint32_t kBackwardBlocks = 2048;
+constexpr uint64_t kBackwardScratchBytes = uint64_t(kBackwardBlocks) * 8 * sizeof(float);
+hipError_t resolve_backward(const lh_scene& s, const lhd::OneView& vs, const lh_resolve_params& params,
+                            const float* grad_radiance, const lh_resolve_grads& grads, hipStream_t st);
 template <class VS>
 hipError_t tonemap(const VS& vs, uint64_t max_pixels, hipStream_t st);
 

@@ -50,6 +50,7 @@ struct f4 {
 
 LH_DEV f3 mk3(const LhVec3& v) { return {v.x, v.y, v.z}; }
 LH_DEV f4 mk4(const LhVec4& v) { return {v.x, v.y, v.z, v.w}; }
+LH_DEV f3 xyz(const LhVec4& v) { return {v.x, v.y, v.z}; }
 LH_DEV f3 add(f3 a, f3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
 LH_DEV f3 sub(f3 a, f3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
 LH_DEV f3 mul(f3 a, float s) { return {a.x * s, a.y * s, a.z * s}; }

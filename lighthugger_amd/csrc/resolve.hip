@@ -25,10 +25,10 @@ using namespace lhd;
 
 namespace {
 
-LH_DEV void resolve_tile(const lh_scene& s, const lh_view& v, uint32_t tx, uint32_t ty) {
+LH_DEV void resolve_tile(const lh_scene& s, const lh_view& v, uint32_t tx, uint32_t ty, const lh_resolve_params& p) {
     const uint32_t x = tx * 8 + (threadIdx.x & 7), y = ty * 8 + (threadIdx.x >> 3);
     if (x >= v.width || y >= v.height) return;
-    const f3 c = shading::resolve_pixel(s, v, x, y);
+    const f3 c = shading::resolve_pixel(s, v, x, y, p);
     const size_t o = size_t(y) * v.width + x;
     reinterpret_cast<float4*>(v.radiance)[o] = make_float4(c.x, c.y, c.z, 1.0f);
     if (v.ldr) v.ldr[o] = shading::tonemap_pixel(c);
@@ -36,11 +36,12 @@ LH_DEV void resolve_tile(const lh_scene& s, const lh_view& v, uint32_t tx, uint3
 
 }  // namespace
 
-__global__ void __launch_bounds__(64) LH_RESOLVE_OCCUPANCY k_resolve(lh_scene s, OneView vs) {
-    resolve_tile(s, vs.v, blockIdx.x, blockIdx.y);
+__global__ void __launch_bounds__(64) LH_RESOLVE_OCCUPANCY k_resolve(lh_scene s, OneView vs, lh_resolve_params p) {
+    resolve_tile(s, vs.v, blockIdx.x, blockIdx.y, p);
 }
 
-__global__ void __launch_bounds__(64) LH_RESOLVE_OCCUPANCY k_resolve_batch(lh_scene s, ViewTable vs) {
+__global__ void __launch_bounds__(64) LH_RESOLVE_OCCUPANCY k_resolve_batch(lh_scene s, ViewTable vs,
+                                                                            lh_resolve_params p) {
     const uint32_t g = blockIdx.x;
     uint32_t lo = 0, hi = vs.n;  // last view with tile_offsets[view] <= g
     while (hi - lo > 1) {
@@ -52,7 +53,7 @@ __global__ void __launch_bounds__(64) LH_RESOLVE_OCCUPANCY k_resolve_batch(lh_sc
     }
     const lh_view& v = vs.views[lo];
     const uint32_t local = g - vs.tile_offsets[lo], tiles_x = (v.width + 7) / 8;
-    resolve_tile(s, v, local % tiles_x, local / tiles_x);
+    resolve_tile(s, v, local % tiles_x, local / tiles_x, p);
 }
 
 template <class VS>
@@ -68,19 +69,19 @@ __global__ void __launch_bounds__(256) k_tonemap(VS vs) {
 namespace lhk {
 
 template <>
-hipError_t resolve<OneView>(const lh_scene& s, const OneView& vs, uint32_t, uint32_t, hipStream_t st) {
+hipError_t resolve<OneView>(const lh_scene& s, const OneView& vs, uint32_t, const lh_resolve_params& p,
+                            hipStream_t st) {
     dim3 grid((vs.v.width + 7) / 8, (vs.v.height + 7) / 8);
     mark("resolve", st);
-    hipLaunchKernelGGL(k_resolve, grid, dim3(64), 0, st, s, vs);
+    hipLaunchKernelGGL(k_resolve, grid, dim3(64), 0, st, s, vs, p);
     return hipGetLastError();
 }
 
-// max_w carries the total tile count of the batch (planned on the host).
 template <>
-hipError_t resolve<ViewTable>(const lh_scene& s, const ViewTable& vs, uint32_t total_tiles, uint32_t,
-                              hipStream_t st) {
+hipError_t resolve<ViewTable>(const lh_scene& s, const ViewTable& vs, uint32_t total_tiles,
+                              const lh_resolve_params& p, hipStream_t st) {
     mark("resolve", st);
-    hipLaunchKernelGGL(k_resolve_batch, dim3(std::max(total_tiles, 1u)), dim3(64), 0, st, s, vs);
+    hipLaunchKernelGGL(k_resolve_batch, dim3(std::max(total_tiles, 1u)), dim3(64), 0, st, s, vs, p);
     return hipGetLastError();
 }
 

@@ -125,22 +125,37 @@ LH_DEV f4 bilinear(const LhTexture& t, uint32_t level, f2 uv) {
                 mix4(texel(t, level, i, j + 1), texel(t, level, i + 1, j + 1), a), b);
 }
 
-// sample_texture (render_geometry.comp:59-66) as isotropic trilinear
-// filtering with the LOD from the analytic uv derivatives.  An unused index
-// reads as opaque white.
-LH_DEV f4 sample_texture(const lh_scene& s, uint32_t index, const Interp2& uv) {
-    if (index == UNUSED_TEXTURE_INDEX || index >= s.num_textures) return {1, 1, 1, 1};
-    const LhTexture& t = s.textures[index];
+// Mip selection of the isotropic trilinear filter: levels l0, l1 and the
+// blend f between them, from the analytic uv derivatives.
+struct Lod {
+    uint32_t l0, l1;
+    float f;
+};
+LH_DEV Lod texture_lod(const LhTexture& t, const Interp2& uv) {
     const float W = float(t.width), H = float(t.height);
     const float ax = uv.dx.x * W, ay = uv.dx.y * H, bx = uv.dy.x * W, by = uv.dy.y * H;
     const float rho = fmaxf(sqrtf(ax * ax + ay * ay), sqrtf(bx * bx + by * by));
     float lambda = rho > 0.0f ? log2f(rho) : 0.0f;
     lambda = clampf(lambda, 0.0f, float(t.levels - 1));
-    const float d0 = floorf(lambda), f = lambda - d0;
-    const uint32_t l0 = uint32_t(d0), l1 = min(l0 + 1, t.levels - 1);
-    const f4 c0 = bilinear(t, l0, uv.v);
-    if (f == 0.0f) return c0;
-    return mix4(c0, bilinear(t, l1, uv.v), f);
+    const float d0 = floorf(lambda);
+    const uint32_t l0 = uint32_t(d0);
+    return {l0, min(l0 + 1, t.levels - 1), lambda - d0};
+}
+
+LH_DEV bool has_texture(const lh_scene& s, uint32_t index) {
+    return index != UNUSED_TEXTURE_INDEX && index < s.num_textures;
+}
+
+// sample_texture (render_geometry.comp:59-66) as isotropic trilinear
+// filtering with the LOD from the analytic uv derivatives.  An unused index
+// reads as opaque white.
+LH_DEV f4 sample_texture(const lh_scene& s, uint32_t index, const Interp2& uv) {
+    if (!has_texture(s, index)) return {1, 1, 1, 1};
+    const LhTexture& t = s.textures[index];
+    const Lod lod = texture_lod(t, uv);
+    const f4 c0 = bilinear(t, lod.l0, uv.v);
+    if (lod.f == 0.0f) return c0;
+    return mix4(c0, bilinear(t, lod.l1, uv.v), lod.f);
 }
 
 // sampler2DArrayShadow tap: linear filter, clamp to edge, compare LESS.
@@ -236,9 +251,60 @@ LH_DEV bool decode(const lh_scene& s, const lh_view& v, uint64_t key, Surface& s
     return true;
 }
 
-LH_DEV f3 resolve_pixel(const lh_scene& s, const lh_view& v, uint32_t x, uint32_t y) {
-    const Uniforms& u = *v.uniforms;
+// Differentiable parameters: each pointer, when set, replaces the value the
+// scene or the uniforms hold (lh_resolve_params).
+LH_DEV f3 param_base_color(const lh_resolve_params& p, const MeshInfo& mi, uint32_t k) {
+    return p.base_color_factor ? f3{p.base_color_factor[3 * k], p.base_color_factor[3 * k + 1],
+                                    p.base_color_factor[3 * k + 2]}
+                               : mk3(mi.base_color_factor);
+}
+LH_DEV float param_roughness(const lh_resolve_params& p, const lh_scene& s, uint32_t k) {
+    return p.roughness_factor ? p.roughness_factor[k] : s.factors[k].roughness_factor;
+}
+LH_DEV float param_metallic(const lh_resolve_params& p, const lh_scene& s, uint32_t k) {
+    return p.metallic_factor ? p.metallic_factor[k] : s.factors[k].metallic_factor;
+}
+LH_DEV f3 param_sun_dir(const lh_resolve_params& p, const Uniforms& u) {
+    return p.sun_dir ? f3{p.sun_dir[0], p.sun_dir[1], p.sun_dir[2]} : mk3(u.sun_dir);
+}
+LH_DEV f3 param_sun_intensity(const lh_resolve_params& p, const Uniforms& u) {
+    return p.sun_intensity ? f3{p.sun_intensity[0], p.sun_intensity[1], p.sun_intensity[2]} : mk3(u.sun_intensity);
+}
+
+// Cascade selection and 3x3 PCF (render_geometry.comp:183-226).
+LH_DEV float shadow_factor(const lh_view& v, const MeshInfo& mi, f3 world) {
     const MiscStorage& misc = *v.misc;
+    uint32_t cascade;
+    f4 sc{0, 0, 0, 0};
+    for (cascade = 0; cascade < 4; cascade++) {
+        sc = mat4_mul_vec4(misc.uv_space_shadow_matrices[cascade], f4{world.x, world.y, world.z, 1.0f});
+        if (fabsf(sc.x - 0.5f) < 0.5f && fabsf(sc.y - 0.5f) < 0.5f) break;
+    }
+    const float bias = (mi.flags & MESH_INFO_FLAGS_ALPHA_CLIP) ? 0.000002f : 0.0f;
+    sc = f4{sc.x / sc.w, sc.y / sc.w, sc.z / sc.w, sc.w / sc.w};
+    const uint32_t layer = min(cascade, 3u);
+    float shadow_sum = 0.0f;
+    const float S = float(v.shadow_size);
+    for (int ox = -1; ox <= 1; ox++)
+        for (int oy = -1; oy <= 1; oy++)
+            shadow_sum += shadow_tap(v, layer, f2{sc.x + float(ox) / S, sc.y + float(oy) / S}, sc.z - bias);
+    shadow_sum /= 9.0f;
+    if (sc.z > 1.0f) shadow_sum = 1.0f;
+    return shadow_sum;
+}
+
+// Spherical-harmonics ambient term of render_geometry.comp:40-57 with its
+// coefficients (only the constant and y bands are non-zero).
+LH_DEV float ambient_sh(f3 nrm) {
+    const float L = 0.025f;
+    return (((((((L + 0.0f * nrm.x) + L * nrm.y) + 0.0f * nrm.z) + 0.0f * (nrm.x * nrm.z)) + 0.0f * (nrm.z * nrm.y)) +
+             0.0f * (nrm.y * nrm.x)) +
+            0.0f * ((3.0f * nrm.z) * nrm.z - 1.0f)) +
+           0.0f * (nrm.x * nrm.x - nrm.y * nrm.y);
+}
+
+LH_DEV f3 resolve_pixel(const lh_scene& s, const lh_view& v, uint32_t x, uint32_t y, const lh_resolve_params& prm) {
+    const Uniforms& u = *v.uniforms;
     const f2 win{float(u.window_size.x), float(u.window_size.y)};
     const f2 ndc{float(x) / win.x * 2.0f - 1.0f, float(y) / win.y * 2.0f - 1.0f};  // pixel corner
     Surface sf;
@@ -263,28 +329,12 @@ LH_DEV f3 resolve_pixel(const lh_scene& s, const lh_view& v, uint32_t x, uint32_
     nrm = normalize3(mat3_mul_vec3(in.normal_transform, nrm));
     const f3 gn = cross3(sub(sf.pos[1], sf.pos[0]), sub(sf.pos[2], sf.pos[0]));
     if (dot3(gn, view_vector) < 0.0f) nrm = neg(nrm);
-    uint32_t cascade;
-    f4 sc{0, 0, 0, 0};
-    for (cascade = 0; cascade < 4; cascade++) {
-        sc = mat4_mul_vec4(misc.uv_space_shadow_matrices[cascade], f4{world.v.x, world.v.y, world.v.z, 1.0f});
-        if (fabsf(sc.x - 0.5f) < 0.5f && fabsf(sc.y - 0.5f) < 0.5f) break;
-    }
-    const float bias = (mi.flags & MESH_INFO_FLAGS_ALPHA_CLIP) ? 0.000002f : 0.0f;
-    sc = f4{sc.x / sc.w, sc.y / sc.w, sc.z / sc.w, sc.w / sc.w};
-    const uint32_t layer = min(cascade, 3u);
-    float shadow_sum = 0.0f;
-    const float S = float(v.shadow_size);
-    for (int ox = -1; ox <= 1; ox++)
-        for (int oy = -1; oy <= 1; oy++)
-            shadow_sum += shadow_tap(v, layer, f2{sc.x + float(ox) / S, sc.y + float(oy) / S}, sc.z - bias);
-    shadow_sum /= 9.0f;
-    if (sc.z > 1.0f) shadow_sum = 1.0f;
-    const LhMaterialFactors fac = s.factors[sf.mesh_info];
+    const float shadow_sum = shadow_factor(v, mi, world.v);
     const f4 bc = sample_texture(s, mi.base_color_texture_index, uv);
-    const f3 base = mul(f3{bc.x, bc.y, bc.z}, mk3(mi.base_color_factor));
+    const f3 base = mul(f3{bc.x, bc.y, bc.z}, param_base_color(prm, mi, sf.mesh_info));
     const f4 mr = sample_texture(s, mi.metallic_roughness_texture_index, uv);
-    const float roughness = mr.y * fac.roughness_factor;
-    const float metallic = mr.z * fac.metallic_factor;
+    const float roughness = mr.y * param_roughness(prm, s, sf.mesh_info);
+    const float metallic = mr.z * param_metallic(prm, s, sf.mesh_info);
     if (mi.normal_texture_index != UNUSED_TEXTURE_INDEX) {
         const f4 mn = sample_texture(s, mi.normal_texture_index, uv);
         const f3 map{mn.x * 255.0f / 127.0f - 128.0f / 127.0f, mn.y * 255.0f / 127.0f - 128.0f / 127.0f,
@@ -298,17 +348,11 @@ LH_DEV f3 resolve_pixel(const lh_scene& s, const lh_view& v, uint32_t x, uint32_
         nrm = normalize3(f3{(t.x * map.x + b.x * map.y) + nrm.x * map.z, (t.y * map.x + b.y * map.y) + nrm.y * map.z,
                             (t.z * map.x + b.z * map.y) + nrm.z * map.z});
     }
-    const float L = 0.025f;
-    const float shv = (((((((L + 0.0f * nrm.x) + L * nrm.y) + 0.0f * nrm.z) + 0.0f * (nrm.x * nrm.z)) +
-                         0.0f * (nrm.z * nrm.y)) +
-                        0.0f * (nrm.y * nrm.x)) +
-                       0.0f * ((3.0f * nrm.z) * nrm.z - 1.0f)) +
-                      0.0f * (nrm.x * nrm.x - nrm.y * nrm.y);
-    const f3 ambient = mul(mul(base, 1.0f - metallic), shv);
-    const f3 sun = mul(mul(brdf(view_vector, mk3(u.sun_dir), nrm, roughness, metallic, base,
+    const f3 ambient = mul(mul(base, 1.0f - metallic), ambient_sh(nrm));
+    const f3 sun = mul(mul(brdf(view_vector, param_sun_dir(prm, u), nrm, roughness, metallic, base,
                                 (mi.flags & MESH_INFO_FLAGS_ALPHA_CLIP) != 0),
                            kPi * shadow_sum),
-                       mk3(u.sun_intensity));
+                       param_sun_intensity(prm, u));
     return add(sun, ambient);
 }
 

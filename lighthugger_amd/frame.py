@@ -111,6 +111,22 @@ class SceneData:
         self.mem.write(self.handle, self.offsets["geometry"] + offset, data)
 
     @property
+    def geometry_base(self) -> int:
+        """Address of the geometry arena (vertex positions, normals, uvs, meshlets)."""
+        return self.base + self.offsets["geometry"]
+
+    @property
+    def geometry_bytes(self) -> int:
+        return int(self.scene.geometry.nbytes)
+
+    def geometry_words(self):
+        """The geometry arena as 32-bit float words, sharing the scene's memory
+        (a torch tensor on the device, a numpy array on the host)."""
+        o, n = self.offsets["geometry"], self.geometry_bytes // 4 * 4
+        t = self.handle[0][self.handle[2] + o: self.handle[2] + o + n]
+        return t.view(np.float32) if self.mem.kind == "host" else t.view(self.mem.torch.float32)
+
+    @property
     def address(self):
         return C.byref(self.c)
 

@@ -50,8 +50,12 @@ def lib() -> C.CDLL:
                                C.c_void_p]
         L.orc_srgb_oetf.restype = C.c_float
         L.orc_srgb_oetf.argtypes = [C.c_float]
-        if hasattr(L, "orc64_resolve_adjoint"):
-            L.orc64_resolve_adjoint.restype = C.c_int32
+        L.orc_resolve_frozen.restype = C.c_int32
+        L.orc_resolve_frozen.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
+        L.orc64_resolve.restype = C.c_int32
+        L.orc64_resolve.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+        L.orc64_resolve_grad.restype = C.c_int32
+        L.orc64_resolve_grad.argtypes = [C.c_void_p] * 5 + [C.c_void_p] * 6
         _lib = L
     return _lib
 

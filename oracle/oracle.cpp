@@ -523,19 +523,28 @@ F4 bilinear(const LhTexture& t, uint32_t level, F2 uv) {
 // from the analytic uv derivatives (the reference sampler asks for 16x
 // anisotropy, src/main.cpp:550-560, which is implementation defined).
 // An unused texture index reads as opaque white.
-F4 sample_texture(const lh_scene& sc, uint32_t index, const Interp2& uv) {
-    if (index == UNUSED_TEXTURE_INDEX || index >= sc.num_textures) return {1, 1, 1, 1};
-    const LhTexture& t = sc.textures[index];
+struct Lod {
+    uint32_t l0, l1;
+    float f;
+};
+Lod texture_lod(const LhTexture& t, const Interp2& uv) {
     float W = float(t.width), H = float(t.height);
     float ax = uv.dx.x * W, ay = uv.dx.y * H, bx = uv.dy.x * W, by = uv.dy.y * H;
     float rho = std::max(std::sqrt(ax * ax + ay * ay), std::sqrt(bx * bx + by * by));
     float lambda = rho > 0.0f ? std::log2(rho) : 0.0f;
     lambda = clampf(lambda, 0.0f, float(t.levels - 1));
-    float d0 = std::floor(lambda), f = lambda - d0;
-    uint32_t l0 = uint32_t(d0), l1 = std::min(l0 + 1, t.levels - 1);
-    F4 c0 = bilinear(t, l0, uv.v);
-    if (f == 0.0f) return c0;
-    return mix4(c0, bilinear(t, l1, uv.v), f);
+    float d0 = std::floor(lambda);
+    uint32_t l0 = uint32_t(d0);
+    return {l0, std::min(l0 + 1, t.levels - 1), lambda - d0};
+}
+bool has_texture(const lh_scene& sc, uint32_t index) { return index != UNUSED_TEXTURE_INDEX && index < sc.num_textures; }
+F4 sample_texture(const lh_scene& sc, uint32_t index, const Interp2& uv) {
+    if (!has_texture(sc, index)) return {1, 1, 1, 1};
+    const LhTexture& t = sc.textures[index];
+    Lod lod = texture_lod(t, uv);
+    F4 c0 = bilinear(t, lod.l0, uv.v);
+    if (lod.f == 0.0f) return c0;
+    return mix4(c0, bilinear(t, lod.l1, uv.v), lod.f);
 }
 
 // One tap of sampler2DArrayShadow with linear filtering, clamp to edge and
@@ -614,6 +623,32 @@ F3 brdf(F3 v, F3 l, F3 n, float pr, float metallic, F3 base, bool thin) {
 // render_geometry.comp:86-101): base colour times 2^3 exposure.
 const F3 kSky{0.35f * 8.0f, 0.50f * 8.0f, 0.80f * 8.0f};
 
+// Cascade selection (render_geometry.comp:183-196; the loop may end at 4,
+// which the array layer clamps to 3) and 3x3 PCF (:206-226).
+float shadow_factor(const lh_view& v, const MeshInfo& mi, F3 world) {
+    const MiscStorage& misc = *v.misc;
+    uint32_t cascade;
+    F4 sc4{0, 0, 0, 0};
+    for (cascade = 0; cascade < 4; cascade++) {
+        sc4 = mat4_mul_vec4(misc.uv_space_shadow_matrices[cascade], F4{world.x, world.y, world.z, 1.0f});
+        float ox = sc4.x - 0.5f, oy = sc4.y - 0.5f;
+        if (std::fabs(ox) < 0.5f && std::fabs(oy) < 0.5f) break;
+    }
+    float bias = (mi.flags & MESH_INFO_FLAGS_ALPHA_CLIP) ? 0.000002f : 0.0f;
+    sc4 = F4{sc4.x / sc4.w, sc4.y / sc4.w, sc4.z / sc4.w, sc4.w / sc4.w};
+    uint32_t layer = std::min(cascade, 3u);
+    float shadow_sum = 0.0f;
+    const float S = float(v.shadow_size);
+    for (int ox = -1; ox <= 1; ox++)
+        for (int oy = -1; oy <= 1; oy++) {
+            F2 o{float(ox) / S, float(oy) / S};
+            shadow_sum += shadow_tap(v, layer, F2{sc4.x + o.x, sc4.y + o.y}, sc4.z - bias);
+        }
+    shadow_sum /= 9.0f;
+    if (sc4.z > 1.0f) shadow_sum = 1.0f;
+    return shadow_sum;
+}
+
 // render_geometry main() (src/shaders/render_geometry.comp:70-266) for one pixel.
 F3 resolve_pixel(const lh_scene& sc, const lh_view& v, uint32_t x, uint32_t y) {
     const Uniforms& u = *v.uniforms;
@@ -654,27 +689,7 @@ F3 resolve_pixel(const lh_scene& sc, const lh_view& v, uint32_t x, uint32_t y) {
     nrm = normalize3(mat3_mul_vec3(in.normal_transform, nrm));
     F3 gn = cross3(sub(pos[1], pos[0]), sub(pos[2], pos[0]));
     if (dot3(gn, view_vector) < 0.0f) nrm = neg(nrm);
-    // Cascade selection (:183-196); the loop may end at 4, which the array
-    // layer clamps to 3.
-    uint32_t cascade;
-    F4 sc4{0, 0, 0, 0};
-    for (cascade = 0; cascade < 4; cascade++) {
-        sc4 = mat4_mul_vec4(misc.uv_space_shadow_matrices[cascade], F4{world.v.x, world.v.y, world.v.z, 1.0f});
-        float ox = sc4.x - 0.5f, oy = sc4.y - 0.5f;
-        if (std::fabs(ox) < 0.5f && std::fabs(oy) < 0.5f) break;
-    }
-    float bias = (mi.flags & MESH_INFO_FLAGS_ALPHA_CLIP) ? 0.000002f : 0.0f;
-    sc4 = F4{sc4.x / sc4.w, sc4.y / sc4.w, sc4.z / sc4.w, sc4.w / sc4.w};
-    uint32_t layer = std::min(cascade, 3u);
-    float shadow_sum = 0.0f;
-    const float S = float(v.shadow_size);
-    for (int ox = -1; ox <= 1; ox++)
-        for (int oy = -1; oy <= 1; oy++) {
-            F2 o{float(ox) / S, float(oy) / S};
-            shadow_sum += shadow_tap(v, layer, F2{sc4.x + o.x, sc4.y + o.y}, sc4.z - bias);
-        }
-    shadow_sum /= 9.0f;
-    if (sc4.z > 1.0f) shadow_sum = 1.0f;
+    float shadow_sum = shadow_factor(v, mi, world.v);
     // Material (:228-246).
     const LhMaterialFactors& fac = sc.factors[(in.mesh_info_address - uint64_t(uintptr_t(sc.mesh_infos))) / sizeof(MeshInfo)];
     F4 bc = sample_texture(sc, mi.base_color_texture_index, uv);
@@ -730,9 +745,68 @@ bool valid(const lh_scene* sc, const lh_view* v) {
            v->height > 0 && v->width <= LH_MAX_VIEW_SIDE && v->height <= LH_MAX_VIEW_SIDE;
 }
 
+// Quantities the differentiable resolve holds fixed, computed in FP32 exactly
+// as resolve_pixel computes them.
+void frozen_pixel(const lh_scene& sc, const lh_view& v, uint32_t x, uint32_t y, orc_frozen& o) {
+    o = orc_frozen{};
+    const Uniforms& u = *v.uniforms;
+    F2 win{float(u.window_size.x), float(u.window_size.y)};
+    F2 ndc{float(x) / win.x * 2.0f - 1.0f, float(y) / win.y * 2.0f - 1.0f};
+    uint64_t key = v.visbuffer[size_t(y) * v.width + x];
+    if (~uint32_t(key >> 32) == 0u) return;  // sky
+    uint32_t mid = uint32_t(key >> LH_KEY_TRI_BITS) & (LH_MAX_SCENE_MESHLETS - 1);
+    uint32_t tri = uint32_t(key) & ((1u << LH_KEY_TRI_BITS) - 1);
+    const MeshletReference& ref = sc.meshlet_table[mid];
+    const Instance& in = sc.instances[ref.instance_index];
+    const MeshInfo& mi = mesh_info_of(in);
+    const Meshlet& ml = meshlet_of(mi, ref.meshlet_index);
+    if (!(mi.flags & MESH_INFO_FLAGS_FP32_VERTICES) || (mi.flags & MESH_INFO_FLAGS_ALPHA_CLIP) ||
+        mi.normal_texture_index != UNUSED_TEXTURE_INDEX) {
+        o.kind = 2;
+        return;
+    }
+    F3 pos[3];
+    F4 clip[3];
+    uint32_t vi[3];
+    for (int j = 0; j < 3; j++) {
+        vi[j] = corner_vertex(mi, ml, tri, j);
+        pos[j] = mat4_point(in.transform, load_position(mi, vi[j]));
+        clip[j] = mat4_mul_vec4(u.combined_perspective_view, F4{pos[j].x, pos[j].y, pos[j].z, 1.0f});
+    }
+    BaryDeriv bary = calc_full_bary(clip[0], clip[1], clip[2], ndc, win);
+    Interp3 world = interp3(bary, pos[0], pos[1], pos[2]);
+    F2 ts{mi.texture_scale.x, mi.texture_scale.y};
+    F2 uvs[3];
+    for (int j = 0; j < 3; j++) {
+        F2 t = load_uv(mi, vi[j]);
+        uvs[j] = {t.x * ts.x, t.y * ts.y};
+    }
+    Interp2 uv = interp2(bary, uvs[0], uvs[1], uvs[2]);
+    F3 view_vector = normalize3(sub(f3(u.camera_pos), world.v));
+    F3 gn = cross3(sub(pos[1], pos[0]), sub(pos[2], pos[0]));
+    o.kind = 1;
+    o.flip = dot3(gn, view_vector) < 0.0f ? -1.0f : 1.0f;
+    o.shadow = shadow_factor(v, mi, world.v);
+    const uint32_t tex[2] = {mi.base_color_texture_index, mi.metallic_roughness_texture_index};
+    for (int k = 0; k < 2; k++) {
+        if (!has_texture(sc, tex[k])) continue;
+        Lod lod = texture_lod(sc.textures[tex[k]], uv);
+        o.lod_l0[k] = lod.l0;
+        o.lod_l1[k] = lod.l1;
+        o.lod_f[k] = lod.f;
+    }
+}
+
 }  // namespace
 
 extern "C" {
+
+lh_status orc_resolve_frozen(const lh_scene* sc, const lh_view* v, orc_frozen* out) {
+    if (!sc || !v || !v->visbuffer || !v->uniforms || !v->misc || !out) return LH_ERR_INVALID;
+    for (uint32_t y = 0; y < v->height; y++)
+        for (uint32_t x = 0; x < v->width; x++) frozen_pixel(*sc, *v, x, y, out[size_t(y) * v->width + x]);
+    return LH_OK;
+}
 
 lh_status orc_view_reset(const lh_scene* sc, const lh_view* v) {
     if (!valid(sc, v)) return LH_ERR_INVALID;

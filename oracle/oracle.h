@@ -63,6 +63,46 @@ void orc_brdf(const float* v, const float* l, const float* n, float roughness, f
 /* sRGB OETF (src/shaders/display_transform.comp:19-25). */
 float orc_srgb_oetf(float x);
 
+/* ---- differentiable resolve ----
+ * Per-pixel quantities the adjoint holds fixed (FP32, as the forward resolve
+ * computes them): kind 0 sky, 1 shaded, 2 unsupported mesh info (16-bit
+ * vertices, alpha clip or a normal texture); back-face flip (+1/-1); PCF
+ * shadow factor; texture LOD of [0] base colour, [1] metallic-roughness. */
+typedef struct orc_frozen {
+    uint32_t kind;
+    float flip;
+    float shadow;
+    uint32_t lod_l0[2];
+    uint32_t lod_l1[2];
+    float lod_f[2];
+} orc_frozen;
+lh_status orc_resolve_frozen(const lh_scene* scene, const lh_view* view, orc_frozen* out /* width*height */);
+
+/* FP64 restatement of the resolve as a function of its differentiable
+ * parameters (each NULL pointer = the scene / uniform value).  geometry, when
+ * set, holds the geometry arena as doubles (one per 4-byte word from
+ * geometry_base) and replaces the FP32 positions and normals. */
+typedef struct orc_params64 {
+    const double* base_color_factor; /* [num_mesh_infos * 3] */
+    const double* roughness_factor;  /* [num_mesh_infos] */
+    const double* metallic_factor;   /* [num_mesh_infos] */
+    const double* sun_dir;           /* [3] */
+    const double* sun_intensity;     /* [3] */
+    const double* geometry;          /* [geometry_bytes / 4] */
+    uint64_t geometry_base;
+    uint64_t geometry_bytes;
+} orc_params64;
+/* FP64 radiance RGB (width*height*3) with the frozen quantities of `frozen`. */
+lh_status orc64_resolve(const lh_scene* scene, const lh_view* view, const orc_params64* params,
+                        const orc_frozen* frozen, double* rgb);
+/* Gradients of sum(grad_radiance . radiance) (grad_radiance: width*height*4
+ * floats, alpha ignored) by forward-mode dual numbers over orc64_resolve,
+ * ADDED into the non-NULL outputs (g_geometry mirrors the geometry arena). */
+lh_status orc64_resolve_grad(const lh_scene* scene, const lh_view* view, const orc_params64* params,
+                             const orc_frozen* frozen, const float* grad_radiance, double* g_base_color,
+                             double* g_roughness, double* g_metallic, double* g_sun_dir, double* g_sun_intensity,
+                             double* g_geometry);
+
 #ifdef __cplusplus
 }
 #endif
