@@ -44,6 +44,10 @@ def parse():
     ap.add_argument("--copy-probe", action="store_true",
                     help="run 256 MiB device-to-device copies (HBM-counter calibration for rocprofv3 --pmc)")
     ap.add_argument("--no-atomic-probe", action="store_true", help="skip the atomicMin ceiling measurement")
+    ap.add_argument("--no-hiz", action="store_true",
+                    help="time the frame without two-phase occlusion culling (output is identical either way)")
+    ap.add_argument("--batch-profile", action="store_true",
+                    help="per-kernel device times of the 64-camera batch, per-view loop vs batched (JSON)")
     return ap.parse_args()
 
 
@@ -247,6 +251,33 @@ def diff_bench(steps: int, tri_scale_pct: int = 18, side: int = 1024) -> dict:
             "unsupported_pixels": unsupported}
 
 
+def batch_profile(scene, n_cameras: int = 64, repeats: int = 3) -> dict:
+    """Per-kernel device time (ms, summed over launches, mean of `repeats`
+    frames) of the batch in both modes (lh_batch_profiled)."""
+    import torch
+
+    from lighthugger_amd import abi
+    from lighthugger_amd.lib import BATCH_BATCHED, BATCH_LOOP, BatchRenderer
+    from lighthugger_amd.scene import batch_cameras
+
+    br = BatchRenderer(scene, batch_cameras(4, n_cameras, 256, 2048), big_capacity=1 << 20)
+    out = {"cameras": n_cameras, "views": 5 * n_cameras}
+    for name, mode in (("loop", BATCH_LOOP), ("batched", BATCH_BATCHED)):
+        br.run(mode)
+        torch.cuda.synchronize()
+        acc: dict[str, float] = {}
+        for _ in range(repeats):
+            for k, ms in br.profiled(mode):
+                acc[k] = acc.get(k, 0.0) + ms / repeats
+        out[name] = {k: round(v, 4) for k, v in sorted(acc.items(), key=lambda kv: -kv[1])}
+        out[name + "_total_ms"] = round(sum(acc.values()), 3)
+    cnt = np.stack([v.counters() for v in br.views]).astype(np.int64)
+    out["surviving_meshlets"] = {"view": int(cnt[:, abi.CNT_DRAWS].sum()),
+                                 "shadow": int(cnt[:, abi.CNT_DRAWS + 1: abi.CNT_DRAWS + 5].sum())}
+    br.close()
+    return out
+
+
 def _tuning_of(cls: int) -> dict:
     import ctypes as C
 
@@ -298,11 +329,14 @@ def main():
     from lighthugger_amd.scene import HostScene, make_uniforms, preset_camera
 
     preset = {"large": abi.PRESET_LARGE, "small": abi.PRESET_SMALL, "stress": abi.PRESET_STRESS}[args.workload]
+    if args.batch_profile:
+        print(json.dumps(batch_profile(HostScene.preset(preset))), flush=True)
+        return
     shadow = {"large": 2048, "small": 1024, "stress": 4096}[args.workload]
     t0 = time.time()
     scene = HostScene.preset(preset)
     cam = preset_camera(preset)
-    cfg = ViewConfig(cam.width, cam.height, shadow, big_capacity=1 << 21, clip_capacity=1 << 18)
+    cfg = ViewConfig(cam.width, cam.height, shadow, big_capacity=1 << 21, clip_capacity=1 << 18, hiz=not args.no_hiz)
     r = Renderer(scene, cfg)
     r.set_camera(cam)
     print(f"# scene built in {time.time() - t0:.1f}s: {scene.num_instances} instances, "
@@ -330,6 +364,20 @@ def main():
         r.frame()
     torch.cuda.synchronize()
     ms = (time.perf_counter() - t_start) * 1000.0 / args.steps
+
+    # the same frame with the occlusion culling switched the other way (same output)
+    r.view.c.flags ^= abi.VIEW_HIZ
+    r.frame()
+    torch.cuda.synchronize()
+    t_alt = time.perf_counter()
+    for _ in range(args.steps):
+        r.frame()
+    torch.cuda.synchronize()
+    ms_alt = (time.perf_counter() - t_alt) * 1000.0 / args.steps
+    r.view.c.flags ^= abi.VIEW_HIZ
+    r.frame()
+    torch.cuda.synchronize()
+    st_hiz = r.status()
 
     # ---- per-kernel device times over the same number of frames ----
     acc: dict[str, list[float]] = {}
@@ -379,6 +427,11 @@ def main():
                 cov_c = int((shadow_maps[c] != np.uint32(abi.LH_SHADOW_CLEAR)).sum())
                 algo += raster_algorithmic_bytes(scene, r.view.draws(1 + c), cov_c, sizes) - 4 * cov_c
             algo //= abi.NUM_CASCADES
+        # the main view launches twice per frame with occlusion culling (two
+        # phases): bytes per launch = the frame's bytes over its launches
+        launches = max(1, round(len(acc[dominant]) / args.steps))
+        if dominant.endswith("_view"):
+            algo //= launches
         achieved = algo / (per_launch[dominant] * 1e-3) / 1e9
         # HBM bytes per launch from the committed rocprofv3 counter runs
         # (FETCH_SIZE, WRITE_SIZE scaled by the copy kernel's), when present
@@ -389,7 +442,8 @@ def main():
             traffic = entry["bytes"] if entry else None
         roofline = {"kernel": dominant, "bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
-                    "algorithmic_bytes": algo, "avg_launch_ms": round(per_launch[dominant], 5)}
+                    "algorithmic_bytes": algo, "avg_launch_ms": round(per_launch[dominant], 5),
+                    "launches_per_frame": len(acc[dominant]) // args.steps}
     else:
         roofline = {"kernel": dominant, "bound": "hbm", "achieved": None, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                     "frac": None, "traffic": None, "avg_launch_ms": round(per_launch[dominant], 5)}
@@ -440,6 +494,12 @@ def main():
         "kernel_ms_per_frame": {k: round(v, 4) for k, v in sorted(per_frame.items(), key=lambda kv: -kv[1])},
         "roofline": roofline,
         "raster_atomics": raster_atomics,
+        "occlusion_culling": {
+            "enabled": cfg.hiz, "ms_per_frame_other_setting": round(ms_alt, 4),
+            "main_view_meshlets_drawn": int(st_hiz[abi.CNT_DRAWS]),
+            "phase1_meshlets": int(st_hiz[abi.CNT_HIZ_PHASE1]) if cfg.hiz else None,
+            "candidates": int(st_hiz[abi.CNT_HIZ_CANDIDATES]) if cfg.hiz else None,
+        },
         "cpu_baseline": cpu,
         "batched_views": batched,
         "differentiable_resolve": differentiable,

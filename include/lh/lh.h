@@ -51,6 +51,8 @@ typedef void* lh_stream; /* hipStream_t */
 #define LH_CNT_CLIP 16     /* + pass: triangles queued for clipping */
 #define LH_CNT_OVERFLOW 24 /* bit mask of LH_OVERFLOW_* */
 #define LH_CNT_FLAGS 25    /* bit mask of LH_FLAG_* */
+#define LH_CNT_HIZ_CANDIDATES 26 /* LH_VIEW_HIZ: small meshlets tested against the depth pyramid */
+#define LH_CNT_HIZ_PHASE1 27     /* LH_VIEW_HIZ: main-view draws rasterized before the pyramid */
 #define LH_CNT_FRAGMENTS 32 /* + pass: covered samples (only with LH_VIEW_STATISTICS) */
 #define LH_CNT_ATOMICS 40   /* + pass: depth atomics issued after the early-out test (same) */
 #define LH_NUM_COUNTERS 64
@@ -63,6 +65,12 @@ typedef void* lh_stream; /* hipStream_t */
 /* lh_view.flags */
 #define LH_VIEW_STATISTICS 0x1u /* single-view raster also counts LH_CNT_FRAGMENTS / LH_CNT_ATOMICS
                                    (slower; batched launches ignore it) */
+#define LH_VIEW_HIZ 0x2u        /* two-phase occlusion culling of the main view (single view only):
+                                   meshlets with a small screen footprint are tested against a depth
+                                   pyramid of what the large ones left and skipped when hidden.  The
+                                   final keys, shadows and radiance are those of the frame without it;
+                                   draws[0] then lists only the meshlets actually rasterized, and the
+                                   candidates are staged in draws[1] (free until the shadow passes). */
 
 /* Bytes per entry of the two raster queues inside lh_view.scratch:
  * [clip queue: clip_capacity x 8 B][pad to 256][big queue: big_capacity x 48 B]. */

@@ -148,6 +148,8 @@ NUM_COUNTERS = 64
 CNT_DRAWS, CNT_BIG, CNT_CLIP, CNT_OVERFLOW, CNT_FLAGS = 0, 8, 16, 24, 25
 CNT_FRAGMENTS, CNT_ATOMICS = 32, 40  # + pass, with VIEW_STATISTICS
 VIEW_STATISTICS = 0x1
+VIEW_HIZ = 0x2
+CNT_HIZ_CANDIDATES, CNT_HIZ_PHASE1 = 26, 27
 FLAG_EMPTY_VIEW, FLAG_BACKWARD_UNSUPPORTED = 0x1, 0x2
 CLIP_ENTRY_BYTES, BIG_ENTRY_BYTES = 8, 48
 LH_OK, LH_ERR_INVALID, LH_ERR_CAPACITY, LH_ERR_HIP, LH_ERR_UNSUPPORTED = 0, 1, 2, 3, 4

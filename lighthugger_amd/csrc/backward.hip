@@ -28,138 +28,177 @@ using namespace shading;
 
 namespace {
 
-
-LH_DEV f3 madd(f3 acc, f3 a, float s) { return add(acc, mul(a, s)); }
-LH_DEV float pow4(float x) {
-    const float x2 = x * x;
+// ---- FP64 vector math (the adjoint's working precision) ----
+struct d2 {
+    double x, y;
+};
+struct d3 {
+    double x, y, z;
+};
+LH_DEV d3 D3(f3 a) { return {double(a.x), double(a.y), double(a.z)}; }
+LH_DEV d3 D3(const LhVec3& a) { return {double(a.x), double(a.y), double(a.z)}; }
+LH_DEV d3 operator+(d3 a, d3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+LH_DEV d3 operator-(d3 a, d3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+LH_DEV d3 operator-(d3 a) { return {-a.x, -a.y, -a.z}; }
+LH_DEV d3 operator*(d3 a, double s) { return {a.x * s, a.y * s, a.z * s}; }
+LH_DEV d3 operator*(d3 a, d3 b) { return {a.x * b.x, a.y * b.y, a.z * b.z}; }
+LH_DEV double dot(d3 a, d3 b) { return (a.x * b.x + a.y * b.y) + a.z * b.z; }
+LH_DEV d3 cross(d3 a, d3 b) { return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x}; }
+LH_DEV double len(d3 a) { return sqrt(dot(a, a)); }
+LH_DEV d3 normalize(d3 a) { return a * (1.0 / len(a)); }
+LH_DEV double clampd(double x, double lo, double hi) { return fmin(fmax(x, lo), hi); }
+LH_DEV double pow4(double x) {
+    const double x2 = x * x;
     return x2 * x2;
 }
+LH_DEV double pow5(double x) { return pow4(x) * x; }
+// columns 0..2 of a 4x4 (column-major) applied to a point / the linear part
+LH_DEV d3 col3(const LhVec4& c) { return {double(c.x), double(c.y), double(c.z)}; }
 
 // u = a / |a|: the gradient of a from the gradient of u.
-LH_DEV f3 normalize_bwd(f3 a, f3 gu) {
-    const float l = length3(a);
-    const f3 u = mul(a, 1.0f / l);
-    return mul(sub(gu, mul(u, dot3(u, gu))), 1.0f / l);
+LH_DEV d3 normalize_bwd(d3 a, d3 gu) {
+    const double l = len(a);
+    const d3 u = a * (1.0 / l);
+    return (gu - u * dot(u, gu)) * (1.0 / l);
 }
 
 struct BrdfGrad {
-    f3 v, l, n, base;
-    float pr, metallic;
+    d3 v, l, n, base;
+    double pr, metallic;
 };
 
-// Reverse mode of shading::brdf (thin = false) for the upstream gradient g of
-// its RGB output.
-LH_DEV void brdf_bwd(f3 v, f3 l, f3 n, float pr, float m, f3 base, f3 g, BrdfGrad& o) {
-    const f3 hv = add(v, l);
-    const f3 h = normalize3(hv);
-    const f3 diffuse = mul(base, 1.0f - m);
-    const float refl = ((1.5f - 1.0f) / (1.5f + 1.0f)) * ((1.5f - 1.0f) / (1.5f + 1.0f));
-    const float d0 = refl * (1.0f - m);
-    const f3 f0 = add(f3{d0, d0, d0}, mul(base, m));
-    const float nv = dot3(n, v), nl = dot3(n, l), nh = dot3(n, h), lh = dot3(l, h);
-    const float NoV = clampf(nv, 1e-5f, 1.0f), NoL = clampf(nl, 0.0f, 1.0f);
-    const float NoH = clampf(nh, 0.0f, 1.0f), LoH = clampf(lh, 0.0f, 1.0f);
-    const float a = pr * pr, a2 = a * a;
-    const float fD = (NoH * a2 - NoH) * NoH + 1.0f;
-    const float D = a2 / ((kPi * fD) * fD);
-    const float p = pow5(1.0f - LoH);
-    const f3 F{f0.x + (1.0f - f0.x) * p, f0.y + (1.0f - f0.y) * p, f0.z + (1.0f - f0.z) * p};
-    const float sL = sqrtf((-NoL * a2 + NoL) * NoL + a2), sV = sqrtf((-NoV * a2 + NoV) * NoV + a2);
-    const float den = NoL * sV + NoV * sL;
-    const float Vis = 0.5f / den;
-    const float f90 = 0.5f + ((2.0f * a) * LoH) * LoH;
-    const float q5L = pow5(1.0f - NoL), q5V = pow5(1.0f - NoV);
-    const float FSL = 1.0f + (f90 - 1.0f) * q5L, FSV = 1.0f + (f90 - 1.0f) * q5V;
-    const float FdB = (FSL * FSV) * (1.0f / kPi);
-    const float DV = D * Vis;
-    const f3 Fr = mul(F, DV), Fd = mul(diffuse, FdB);
-    const f3 irr = mul(add(Fd, Fr), NoL);
+// BRDF of shading::brdf (thin = false) in FP64: value and reverse mode for the
+// upstream gradient g of its RGB output.
+LH_DEV d3 brdf_bwd(d3 v, d3 l, d3 n, double pr, double m, d3 base, d3 g, BrdfGrad& o) {
+    const d3 hv = v + l;
+    const d3 h = normalize(hv);
+    const d3 diffuse = base * (1.0 - m);
+    const double refl = (0.5 / 2.5) * (0.5 / 2.5);
+    const double d0 = refl * (1.0 - m);
+    const d3 f0 = d3{d0, d0, d0} + base * m;
+    const double nv = dot(n, v), nl = dot(n, l), nh = dot(n, h), lh = dot(l, h);
+    const double NoV = clampd(nv, 1e-5, 1.0), NoL = clampd(nl, 0.0, 1.0);
+    const double NoH = clampd(nh, 0.0, 1.0), LoH = clampd(lh, 0.0, 1.0);
+    const double a = pr * pr, a2 = a * a;
+    const double fD = (NoH * a2 - NoH) * NoH + 1.0;
+    const double D = a2 / ((M_PI * fD) * fD);
+    const double p = pow5(1.0 - LoH);
+    const d3 F{f0.x + (1.0 - f0.x) * p, f0.y + (1.0 - f0.y) * p, f0.z + (1.0 - f0.z) * p};
+    const double sL = sqrt((-NoL * a2 + NoL) * NoL + a2), sV = sqrt((-NoV * a2 + NoV) * NoV + a2);
+    const double den = NoL * sV + NoV * sL;
+    const double Vis = 0.5 / den;
+    const double f90 = 0.5 + ((2.0 * a) * LoH) * LoH;
+    const double q5L = pow5(1.0 - NoL), q5V = pow5(1.0 - NoV);
+    const double FSL = 1.0 + (f90 - 1.0) * q5L, FSV = 1.0 + (f90 - 1.0) * q5V;
+    const double FdB = (FSL * FSV) * (1.0 / M_PI);
+    const double DV = D * Vis;
+    const d3 Fr = F * DV, Fd = diffuse * FdB;
+    const d3 irr = (Fd + Fr) * NoL;
     // max(irr, 0) and the NaN guard pass the gradient only where irr > 0
-    const f3 gi{irr.x > 0.0f ? g.x : 0.0f, irr.y > 0.0f ? g.y : 0.0f, irr.z > 0.0f ? g.z : 0.0f};
-    const f3 gs = mul(gi, NoL);
-    float gNoL = dot3(gi, add(Fd, Fr));
-    const f3 gF = mul(gs, DV);
-    const float gDV = dot3(gs, F);
-    const float gD = gDV * Vis, gVis = gDV * D;
-    const f3 gdiff = mul(gs, FdB);
-    const float gFdB = dot3(gs, diffuse);
-    const float gFSL = gFdB * FSV * (1.0f / kPi), gFSV = gFdB * FSL * (1.0f / kPi);
-    const float gf90 = gFSL * q5L + gFSV * q5V;
-    gNoL += gFSL * (f90 - 1.0f) * (-5.0f * pow4(1.0f - NoL));
-    float gNoV = gFSV * (f90 - 1.0f) * (-5.0f * pow4(1.0f - NoV));
-    float ga = gf90 * (2.0f * LoH * LoH);
-    float gLoH = gf90 * (4.0f * a * LoH);
-    const float gden = -gVis * Vis / den;
+    const d3 gi{irr.x > 0.0 ? g.x : 0.0, irr.y > 0.0 ? g.y : 0.0, irr.z > 0.0 ? g.z : 0.0};
+    const d3 gs = gi * NoL;
+    double gNoL = dot(gi, Fd + Fr);
+    const d3 gF = gs * DV;
+    const double gDV = dot(gs, F);
+    const double gD = gDV * Vis, gVis = gDV * D;
+    const d3 gdiff = gs * FdB;
+    const double gFdB = dot(gs, diffuse);
+    const double gFSL = gFdB * FSV * (1.0 / M_PI), gFSV = gFdB * FSL * (1.0 / M_PI);
+    const double gf90 = gFSL * q5L + gFSV * q5V;
+    gNoL += gFSL * (f90 - 1.0) * (-5.0 * pow4(1.0 - NoL));
+    double gNoV = gFSV * (f90 - 1.0) * (-5.0 * pow4(1.0 - NoV));
+    double ga = gf90 * (2.0 * LoH * LoH);
+    double gLoH = gf90 * (4.0 * a * LoH);
+    const double gden = -gVis * Vis / den;
     gNoV += gden * sL;
     gNoL += gden * sV;
-    const float gsL = gden * NoV, gsV = gden * NoL;
-    gNoL += gsL * NoL * (1.0f - a2) / sL;
-    gNoV += gsV * NoV * (1.0f - a2) / sV;
-    float ga2 = gsL * (1.0f - NoL * NoL) / (2.0f * sL) + gsV * (1.0f - NoV * NoV) / (2.0f * sV);
-    const f3 gf0{gF.x * (1.0f - p), gF.y * (1.0f - p), gF.z * (1.0f - p)};
-    const float gp = (gF.x * (1.0f - f0.x) + gF.y * (1.0f - f0.y)) + gF.z * (1.0f - f0.z);
-    gLoH += gp * (-5.0f * pow4(1.0f - LoH));
-    const float E = D / fD;
-    ga2 += gD * (1.0f / ((kPi * fD) * fD) - 2.0f * E * NoH * NoH);
-    const float gNoH = gD * (-2.0f * E * (2.0f * NoH * (a2 - 1.0f)));
-    ga += ga2 * 2.0f * a;
-    o.pr = ga * 2.0f * pr;
-    o.metallic = dot3(gf0, sub(base, f3{refl, refl, refl})) - dot3(gdiff, base);
-    o.base = add(mul(gf0, m), mul(gdiff, 1.0f - m));
-    const float gnv = (nv > 1e-5f && nv < 1.0f) ? gNoV : 0.0f;
-    const float gnl = (nl > 0.0f && nl < 1.0f) ? gNoL : 0.0f;
-    const float gnh = (nh > 0.0f && nh < 1.0f) ? gNoH : 0.0f;
-    const float glh = (lh > 0.0f && lh < 1.0f) ? gLoH : 0.0f;
-    o.n = add(add(mul(v, gnv), mul(l, gnl)), mul(h, gnh));
-    o.v = mul(n, gnv);
-    o.l = add(mul(n, gnl), mul(h, glh));
-    const f3 ghv = normalize_bwd(hv, add(mul(n, gnh), mul(l, glh)));
-    o.v = add(o.v, ghv);
-    o.l = add(o.l, ghv);
+    const double gsL = gden * NoV, gsV = gden * NoL;
+    gNoL += gsL * NoL * (1.0 - a2) / sL;
+    gNoV += gsV * NoV * (1.0 - a2) / sV;
+    double ga2 = gsL * (1.0 - NoL * NoL) / (2.0 * sL) + gsV * (1.0 - NoV * NoV) / (2.0 * sV);
+    const d3 gf0 = gF * (1.0 - p);
+    const double gp = (gF.x * (1.0 - f0.x) + gF.y * (1.0 - f0.y)) + gF.z * (1.0 - f0.z);
+    gLoH += gp * (-5.0 * pow4(1.0 - LoH));
+    const double E = D / fD;
+    ga2 += gD * (1.0 / ((M_PI * fD) * fD) - 2.0 * E * NoH * NoH);
+    const double gNoH = gD * (-2.0 * E * (2.0 * NoH * (a2 - 1.0)));
+    ga += ga2 * 2.0 * a;
+    o.pr = ga * 2.0 * pr;
+    o.metallic = dot(gf0, base - d3{refl, refl, refl}) - dot(gdiff, base);
+    o.base = gf0 * m + gdiff * (1.0 - m);
+    const double gnv = (nv > 1e-5 && nv < 1.0) ? gNoV : 0.0;
+    const double gnl = (nl > 0.0 && nl < 1.0) ? gNoL : 0.0;
+    const double gnh = (nh > 0.0 && nh < 1.0) ? gNoH : 0.0;
+    const double glh = (lh > 0.0 && lh < 1.0) ? gLoH : 0.0;
+    o.n = (v * gnv + l * gnl) + h * gnh;
+    const d3 ghv = normalize_bwd(hv, n * gnh + l * glh);
+    o.v = n * gnv + ghv;
+    o.l = (n * gnl + h * glh) + ghv;
+    return {irr.x > 0.0 ? irr.x : 0.0, irr.y > 0.0 ? irr.y : 0.0, irr.z > 0.0 ? irr.z : 0.0};
 }
 
-// d(bilinear)/d(uv) at one level, contracted with the upstream gradient g.
-LH_DEV f2 bilinear_bwd(const LhTexture& t, uint32_t level, f2 uv, f4 g) {
-    const float w = float(max(1u, t.width >> level)), h = float(max(1u, t.height >> level));
-    const float u = uv.x * w - 0.5f, v = uv.y * h - 0.5f;
-    const float fu = floorf(u), fv = floorf(v);
-    const float a = u - fu, b = v - fv;
+// ---- textures in FP64 with the forward's (FP32) LOD held fixed ----
+struct Texel4 {
+    double c[4];
+};
+LH_DEV Texel4 texel64(const LhTexture& t, uint32_t level, int32_t i, int32_t j) {
+    const int32_t w = int32_t(max(1u, t.width >> level)), h = int32_t(max(1u, t.height >> level));
+    i %= w;
+    if (i < 0) i += w;
+    j %= h;
+    if (j < 0) j += h;
+    const uint32_t q = ptr<uint32_t>(t.texels)[size_t(t.level_offset[level]) + size_t(j) * w + i];
+    return {{double(q & 255u) / 255.0, double((q >> 8) & 255u) / 255.0, double((q >> 16) & 255u) / 255.0,
+             double(q >> 24) / 255.0}};
+}
+// Bilinear value (val) and d/d(uv) contracted with g, at one level.
+LH_DEV void bilinear64(const LhTexture& t, uint32_t level, d2 uv, const double* g, double* val, d2& guv) {
+    const double w = double(max(1u, t.width >> level)), h = double(max(1u, t.height >> level));
+    const double u = uv.x * w - 0.5, vv = uv.y * h - 0.5;
+    const double fu = floor(u), fv = floor(vv);
+    const double a = u - fu, b = vv - fv;
     const int32_t i = int32_t(fu), j = int32_t(fv);
-    const f4 t00 = texel(t, level, i, j), t10 = texel(t, level, i + 1, j);
-    const f4 t01 = texel(t, level, i, j + 1), t11 = texel(t, level, i + 1, j + 1);
-    auto da = [&](float x00, float x10, float x01, float x11) { return (1.0f - b) * (x10 - x00) + b * (x11 - x01); };
-    auto db = [&](float x00, float x10, float x01, float x11) {
-        return ((1.0f - a) * x01 + a * x11) - ((1.0f - a) * x00 + a * x10);
-    };
-    const float ga = ((g.x * da(t00.x, t10.x, t01.x, t11.x) + g.y * da(t00.y, t10.y, t01.y, t11.y)) +
-                      g.z * da(t00.z, t10.z, t01.z, t11.z)) + g.w * da(t00.w, t10.w, t01.w, t11.w);
-    const float gb = ((g.x * db(t00.x, t10.x, t01.x, t11.x) + g.y * db(t00.y, t10.y, t01.y, t11.y)) +
-                      g.z * db(t00.z, t10.z, t01.z, t11.z)) + g.w * db(t00.w, t10.w, t01.w, t11.w);
-    return {ga * w, gb * h};
+    const Texel4 t00 = texel64(t, level, i, j), t10 = texel64(t, level, i + 1, j);
+    const Texel4 t01 = texel64(t, level, i, j + 1), t11 = texel64(t, level, i + 1, j + 1);
+    double ga = 0.0, gb = 0.0;
+    for (int k = 0; k < 4; k++) {
+        const double top = t00.c[k] * (1.0 - a) + t10.c[k] * a, bot = t01.c[k] * (1.0 - a) + t11.c[k] * a;
+        val[k] = top * (1.0 - b) + bot * b;
+        ga += g[k] * ((1.0 - b) * (t10.c[k] - t00.c[k]) + b * (t11.c[k] - t01.c[k]));
+        gb += g[k] * (bot - top);
+    }
+    guv = {ga * w, gb * h};
 }
-
-// Gradient wrt uv of the trilinear sample (LOD held fixed).
-LH_DEV f2 sample_texture_bwd(const lh_scene& s, uint32_t index, const Interp2& uv, f4 g) {
-    if (!has_texture(s, index)) return {0.0f, 0.0f};
+// Trilinear sample with the frozen LOD; the uv gradient is for the upstream g.
+// Unused index: white, no gradient.
+LH_DEV void sample64(const lh_scene& s, uint32_t index, const Lod& lod, d2 uv, const double* g, double* val, d2& guv) {
+    guv = {0.0, 0.0};
+    if (!has_texture(s, index)) {
+        for (int k = 0; k < 4; k++) val[k] = 1.0;
+        return;
+    }
     const LhTexture& t = s.textures[index];
-    const Lod lod = texture_lod(t, uv);
-    f2 r = bilinear_bwd(t, lod.l0, uv.v, g);
-    if (lod.f == 0.0f) return r;
-    const f2 r1 = bilinear_bwd(t, lod.l1, uv.v, g);
-    return {r.x * (1.0f - lod.f) + r1.x * lod.f, r.y * (1.0f - lod.f) + r1.y * lod.f};
+    bilinear64(t, lod.l0, uv, g, val, guv);
+    if (lod.f == 0.0f) return;
+    double v1[4];
+    d2 g1;
+    bilinear64(t, lod.l1, uv, g, v1, g1);
+    const double f = lod.f;
+    for (int k = 0; k < 4; k++) val[k] = val[k] * (1.0 - f) + v1[k] * f;
+    guv = {guv.x * (1.0 - f) + g1.x * f, guv.y * (1.0 - f) + g1.y * f};
 }
 
 struct LightGrad {
-    f3 sun_dir, sun_intensity;
+    d3 sun_dir, sun_intensity;
 };
 
-LH_DEV void add_geometry(const lh_resolve_grads& gr, uint64_t addr, f3 g) {
+LH_DEV void add_geometry(const lh_resolve_grads& gr, uint64_t addr, d3 g) {
     if (!gr.geometry || addr < gr.geometry_base) return;
     const uint64_t k = (addr - gr.geometry_base) / 4;
     if (k + 3 > gr.geometry_bytes / 4) return;
-    atomicAdd(gr.geometry + k, g.x);
-    atomicAdd(gr.geometry + k + 1, g.y);
-    atomicAdd(gr.geometry + k + 2, g.z);
+    atomicAdd(gr.geometry + k, float(g.x));
+    atomicAdd(gr.geometry + k + 1, float(g.y));
+    atomicAdd(gr.geometry + k + 2, float(g.z));
 }
 
 LH_DEV bool supported(const MeshInfo& mi) {
@@ -167,12 +206,42 @@ LH_DEV bool supported(const MeshInfo& mi) {
            mi.normal_texture_index == UNUSED_TEXTURE_INDEX;
 }
 
+// Quantities held fixed, computed in FP32 exactly as shading::resolve_pixel.
+struct Frozen {
+    float flip, K;
+    Lod lod_b, lod_m;
+};
+LH_DEV Frozen frozen(const lh_scene& s, const lh_view& v, const MeshInfo& mi, const Surface& sf, f2 ndc, f2 win) {
+    const Uniforms& u = *v.uniforms;
+    const BaryDeriv bary = calc_full_bary(sf.clip[0], sf.clip[1], sf.clip[2], ndc, win);
+    const Interp3 world = interp3(bary, sf.pos[0], sf.pos[1], sf.pos[2]);
+    const f2 ts{mi.texture_scale.x, mi.texture_scale.y};
+    f2 uvs[3];
+    for (int j = 0; j < 3; j++) {
+        const f2 t = load_uv(mi, sf.vi[j]);
+        uvs[j] = {t.x * ts.x, t.y * ts.y};
+    }
+    const Interp2 uv = interp2(bary, uvs[0], uvs[1], uvs[2]);  // derivatives only: the offset does not matter
+    const f3 view_vector = normalize3(sub(mk3(u.camera_pos), world.v));
+    const f3 gn = cross3(sub(sf.pos[1], sf.pos[0]), sub(sf.pos[2], sf.pos[0]));
+    Frozen fz;
+    fz.flip = dot3(gn, view_vector) < 0.0f ? -1.0f : 1.0f;
+    fz.K = kPi * shadow_factor(v, mi, world.v);
+    fz.lod_b = has_texture(s, mi.base_color_texture_index) ? texture_lod(s.textures[mi.base_color_texture_index], uv)
+                                                           : Lod{0, 0, 0.0f};
+    fz.lod_m = has_texture(s, mi.metallic_roughness_texture_index)
+                   ? texture_lod(s.textures[mi.metallic_roughness_texture_index], uv)
+                   : Lod{0, 0, 0.0f};
+    return fz;
+}
+
 // One pixel: scatters vertex gradients, returns its material-factor gradients
-// (mesh info in *mesh_info, or ~0u when it contributes nothing) and adds its
-// light gradients to lg.
+// (mesh info in mesh_info, or ~0u when it contributes nothing) and adds its
+// light gradients to lg.  The function and its reverse are evaluated in FP64
+// from the FP32 inputs.
 LH_DEV void pixel_bwd(const lh_scene& s, const lh_view& v, const lh_resolve_params& prm, const float* grad_radiance,
-                      const lh_resolve_grads& gr, uint32_t x, uint32_t y, uint32_t& mesh_info, f3& g_bcf, float& g_rf,
-                      float& g_mf, LightGrad& lg) {
+                      const lh_resolve_grads& gr, uint32_t x, uint32_t y, uint32_t& mesh_info, d3& g_bcf, double& g_rf,
+                      double& g_mf, LightGrad& lg) {
     mesh_info = ~0u;
     const size_t o = size_t(y) * v.width + x;
     Surface sf;
@@ -184,124 +253,131 @@ LH_DEV void pixel_bwd(const lh_scene& s, const lh_view& v, const lh_resolve_para
         atomicOr(&v.counters[LH_CNT_FLAGS], LH_FLAG_BACKWARD_UNSUPPORTED);
         return;
     }
-    const float4 gin = reinterpret_cast<const float4*>(grad_radiance)[o];
-    const f3 g{gin.x, gin.y, gin.z};
-    // ---- forward recomputation (as shading::resolve_pixel) ----
     const f2 win{float(u.window_size.x), float(u.window_size.y)};
     const f2 ndc{float(x) / win.x * 2.0f - 1.0f, float(y) / win.y * 2.0f - 1.0f};
-    const BaryDeriv bary = calc_full_bary(sf.clip[0], sf.clip[1], sf.clip[2], ndc, win);
-    const Interp3 world = interp3(bary, sf.pos[0], sf.pos[1], sf.pos[2]);
-    const f2 ts{mi.texture_scale.x, mi.texture_scale.y};
-    f2 uvs[3];
-    f3 nraw[3], nn[3];
-    for (int j = 0; j < 3; j++) {
-        const f2 t = load_uv(mi, sf.vi[j]);
-        uvs[j] = {t.x * ts.x, t.y * ts.y};
-        nraw[j] = load_normal(mi, sf.vi[j]);
-        nn[j] = normalize3(nraw[j]);
-    }
-    Interp2 uv = interp2(bary, uvs[0], uvs[1], uvs[2]);
-    uv.v.x += mi.texture_offset.x;
-    uv.v.y += mi.texture_offset.y;
-    const f3 cam = mk3(u.camera_pos);
-    const f3 to_cam = sub(cam, world.v);
-    const f3 view_vector = normalize3(to_cam);
-    const f3 nI = interp3(bary, nn[0], nn[1], nn[2]).v;
-    const f3 m3 = mat3_mul_vec3(in.normal_transform, nI);
-    f3 nrm = normalize3(m3);
-    const f3 gn = cross3(sub(sf.pos[1], sf.pos[0]), sub(sf.pos[2], sf.pos[0]));
-    const float flip = dot3(gn, view_vector) < 0.0f ? -1.0f : 1.0f;
-    nrm = mul(nrm, flip);
-    const float shadow_sum = shadow_factor(v, mi, world.v);
-    const f4 bc = sample_texture(s, mi.base_color_texture_index, uv);
-    const f3 bcf = param_base_color(prm, mi, sf.mesh_info);
-    const f3 base = mul(f3{bc.x, bc.y, bc.z}, bcf);
-    const f4 mr = sample_texture(s, mi.metallic_roughness_texture_index, uv);
-    const float rf = param_roughness(prm, s, sf.mesh_info), mf = param_metallic(prm, s, sf.mesh_info);
-    const float roughness = mr.y * rf, metallic = mr.z * mf;
-    const float shv = ambient_sh(nrm);
-    const f3 sun_dir = param_sun_dir(prm, u), sun_int = param_sun_intensity(prm, u);
-    const f3 B = brdf(view_vector, sun_dir, nrm, roughness, metallic, base, false);
-    const float K = kPi * shadow_sum;
-    // ---- reverse ----
-    // out = B * K * I + base * (1 - metallic) * shv
-    lg.sun_intensity = add(lg.sun_intensity, mul(mul(g, B), K));
-    const f3 gB = mul(mul(g, sun_int), K);
-    f3 gbase = mul(g, (1.0f - metallic) * shv);
-    float gmet = -dot3(g, base) * shv;
-    const float gshv = dot3(g, base) * (1.0f - metallic);
-    BrdfGrad bg;
-    brdf_bwd(view_vector, sun_dir, nrm, roughness, metallic, base, gB, bg);
-    lg.sun_dir = add(lg.sun_dir, bg.l);
-    gbase = add(gbase, bg.base);
-    gmet += bg.metallic;
-    f3 gnrm = bg.n;
-    gnrm.y += gshv * 0.025f;  // d(ambient_sh)/d(nrm.y)
-    // factors and texture samples
-    mesh_info = sf.mesh_info;
-    g_bcf = mul(gbase, f3{bc.x, bc.y, bc.z});
-    g_rf = bg.pr * mr.y;
-    g_mf = gmet * mr.z;
-    const f2 guv_b = sample_texture_bwd(s, mi.base_color_texture_index, uv, f4{gbase.x * bcf.x, gbase.y * bcf.y,
-                                                                               gbase.z * bcf.z, 0.0f});
-    const f2 guv_m = sample_texture_bwd(s, mi.metallic_roughness_texture_index, uv, f4{0.0f, bg.pr * rf, gmet * mf, 0.0f});
-    const f2 guv{guv_b.x + guv_m.x, guv_b.y + guv_m.y};
-    // normal chain: nrm = flip * normalize(N nI), nI = sum lambda_j normalize(nraw_j)
-    const f3 gm3 = normalize_bwd(m3, mul(gnrm, flip));
-    const LhMat3& N = in.normal_transform;
-    const f3 gnI{dot3(mk3(N.c[0]), gm3), dot3(mk3(N.c[1]), gm3), dot3(mk3(N.c[2]), gm3)};
-    // world position: view vector (and nothing else: shadow and LOD are fixed)
-    const f3 gworld = neg(normalize_bwd(to_cam, bg.v));
-    const float lam[3] = {bary.lambda.x, bary.lambda.y, bary.lambda.z};
-    float glam[3];
-    f3 gpos[3];
-    for (int j = 0; j < 3; j++) {
-        glam[j] = (dot3(gworld, sf.pos[j]) + dot3(gnI, nn[j])) + (guv.x * uvs[j].x + guv.y * uvs[j].y);
-        gpos[j] = mul(gworld, lam[j]);
-        add_geometry(gr, mi.normals + 12ull * sf.vi[j], normalize_bwd(nraw[j], mul(gnI, lam[j])));
-    }
-    // barycentrics: lambda_j = D_j / sum D, D_j = q . (c_{j+1} x c_{j+2})
-    const f3 q{ndc.x, ndc.y, 1.0f};
-    f3 c[3];
-    for (int j = 0; j < 3; j++) c[j] = f3{sf.clip[j].x, sf.clip[j].y, sf.clip[j].w};
-    float Dk[3];
-    for (int k = 0; k < 3; k++) Dk[k] = dot3(q, cross3(c[(k + 1) % 3], c[(k + 2) % 3]));
-    const float sumD = (Dk[0] + Dk[1]) + Dk[2];
-    const float lg_dot = (glam[0] * Dk[0] + glam[1] * Dk[1] + glam[2] * Dk[2]) / sumD;
-    f3 gc[3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
-    for (int k = 0; k < 3; k++) {
-        const float gD = (glam[k] - lg_dot) / sumD;
-        const int a = (k + 1) % 3, b = (k + 2) % 3;
-        gc[a] = madd(gc[a], cross3(c[b], q), gD);
-        gc[b] = madd(gc[b], cross3(q, c[a]), gD);
-    }
-    const LhMat4& PV = u.combined_perspective_view;
+    const Frozen fz = frozen(s, v, mi, sf, ndc, win);
+    const float4 gin = reinterpret_cast<const float4*>(grad_radiance)[o];
+    const d3 g{gin.x, gin.y, gin.z};
+    // ---- forward in FP64 ----
     const LhMat4& T = in.transform;
+    const LhMat4& PV = u.combined_perspective_view;
+    d3 pos[3], c[3], nraw[3], nn[3];
+    d2 uvs[3];
     for (int j = 0; j < 3; j++) {
-        const f4 gclip{gc[j].x, gc[j].y, 0.0f, gc[j].z};
+        const d3 L = D3(sf.local[j]);
+        pos[j] = ((col3(T.c[0]) * L.x + col3(T.c[1]) * L.y) + col3(T.c[2]) * L.z) + col3(T.c[3]);
+        // clip (x, y, w) = rows 0, 1, 3 of PV [pos, 1]
+        auto row = [&](int r) {
+            auto e = [&](int k) {
+                const LhVec4& cc = PV.c[k];
+                return double(r == 0 ? cc.x : r == 1 ? cc.y : cc.w);
+            };
+            return ((e(0) * pos[j].x + e(1) * pos[j].y) + e(2) * pos[j].z) + e(3);
+        };
+        c[j] = {row(0), row(1), row(3)};
+        nraw[j] = D3(load_normal(mi, sf.vi[j]));
+        nn[j] = normalize(nraw[j]);
+        const f2 t = load_uv(mi, sf.vi[j]);
+        uvs[j] = {double(t.x) * double(mi.texture_scale.x), double(t.y) * double(mi.texture_scale.y)};
+    }
+    // perspective-correct barycentrics: lambda_j = D_j / sum D, D_j = q . (c_{j+1} x c_{j+2})
+    const d3 q{double(x) / double(u.window_size.x) * 2.0 - 1.0, double(y) / double(u.window_size.y) * 2.0 - 1.0, 1.0};
+    double Dk[3];
+    for (int k = 0; k < 3; k++) Dk[k] = dot(q, cross(c[(k + 1) % 3], c[(k + 2) % 3]));
+    const double sumD = (Dk[0] + Dk[1]) + Dk[2];
+    const double lam[3] = {Dk[0] / sumD, Dk[1] / sumD, Dk[2] / sumD};
+    const d3 world = (pos[0] * lam[0] + pos[1] * lam[1]) + pos[2] * lam[2];
+    const d2 uv{((lam[0] * uvs[0].x + lam[1] * uvs[1].x) + lam[2] * uvs[2].x) + double(mi.texture_offset.x),
+                ((lam[0] * uvs[0].y + lam[1] * uvs[1].y) + lam[2] * uvs[2].y) + double(mi.texture_offset.y)};
+    const d3 to_cam = D3(u.camera_pos) - world;
+    const d3 V = normalize(to_cam);
+    const d3 nI = (nn[0] * lam[0] + nn[1] * lam[1]) + nn[2] * lam[2];
+    const LhMat3& N = in.normal_transform;
+    const d3 m3 = (D3(N.c[0]) * nI.x + D3(N.c[1]) * nI.y) + D3(N.c[2]) * nI.z;
+    const d3 nrm = normalize(m3) * double(fz.flip);
+    const d3 bcf = D3(param_base_color(prm, mi, sf.mesh_info));
+    const double rf = param_roughness(prm, s, sf.mesh_info), mf = param_metallic(prm, s, sf.mesh_info);
+    const d3 sun_dir = D3(param_sun_dir(prm, u)), sun_int = D3(param_sun_intensity(prm, u));
+    // the texture samples are needed before their upstream gradients are known:
+    // take the values first (zero upstream), the uv gradients after the reverse
+    const double zero4[4] = {0.0, 0.0, 0.0, 0.0};
+    double bc[4], mr[4];
+    d2 unused;
+    sample64(s, mi.base_color_texture_index, fz.lod_b, uv, zero4, bc, unused);
+    sample64(s, mi.metallic_roughness_texture_index, fz.lod_m, uv, zero4, mr, unused);
+    const d3 base = d3{bc[0], bc[1], bc[2]} * bcf;
+    const double roughness = mr[1] * rf, metallic = mr[2] * mf;
+    const double shv = nrm.y * 0.025 + 0.025;  // SH ambient: constant and y bands
+    const double K = fz.K;
+    // ---- reverse ----
+    // out = B * K * I + base * (1 - metallic) * shv;  B needs the upstream gB = g * I * K
+    BrdfGrad bg;
+    const d3 B = brdf_bwd(V, sun_dir, nrm, roughness, metallic, base, g * sun_int * K, bg);
+    lg.sun_intensity = lg.sun_intensity + g * B * K;
+    lg.sun_dir = lg.sun_dir + bg.l;
+    const d3 gbase = g * ((1.0 - metallic) * shv) + bg.base;
+    const double gmet = -dot(g, base) * shv + bg.metallic;
+    d3 gnrm = bg.n;
+    gnrm.y += dot(g, base) * (1.0 - metallic) * 0.025;
+    mesh_info = sf.mesh_info;
+    g_bcf = gbase * d3{bc[0], bc[1], bc[2]};
+    g_rf = bg.pr * mr[1];
+    g_mf = gmet * mr[2];
+    const double gb4[4] = {gbase.x * bcf.x, gbase.y * bcf.y, gbase.z * bcf.z, 0.0};
+    const double gm4[4] = {0.0, bg.pr * rf, gmet * mf, 0.0};
+    double tmp[4];
+    d2 guv_b, guv_m;
+    sample64(s, mi.base_color_texture_index, fz.lod_b, uv, gb4, tmp, guv_b);
+    sample64(s, mi.metallic_roughness_texture_index, fz.lod_m, uv, gm4, tmp, guv_m);
+    const d2 guv{guv_b.x + guv_m.x, guv_b.y + guv_m.y};
+    // normal chain: nrm = flip * normalize(N nI), nI = sum lambda_j normalize(nraw_j)
+    const d3 gm3 = normalize_bwd(m3, gnrm * double(fz.flip));
+    const d3 gnI{dot(D3(N.c[0]), gm3), dot(D3(N.c[1]), gm3), dot(D3(N.c[2]), gm3)};
+    // world position through the view vector (shadow and LOD are fixed)
+    const d3 gworld = -normalize_bwd(to_cam, bg.v);
+    // positions relative to vertex 0: lambda sums to 1, so a common shift of
+    // the lambda gradients cancels and the relative form is better conditioned
+    double glam[3];
+    d3 gpos[3];
+    for (int j = 0; j < 3; j++) {
+        glam[j] = (dot(gworld, pos[j] - pos[0]) + dot(gnI, nn[j])) +
+                  (guv.x * (uvs[j].x - uvs[0].x) + guv.y * (uvs[j].y - uvs[0].y));
+        gpos[j] = gworld * lam[j];
+        add_geometry(gr, mi.normals + 12ull * sf.vi[j], normalize_bwd(nraw[j], gnI * lam[j]));
+    }
+    const double lg_dot = (glam[0] * lam[0] + glam[1] * lam[1]) + glam[2] * lam[2];
+    d3 gc[3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+    for (int k = 0; k < 3; k++) {
+        const double gD = (glam[k] - lg_dot) / sumD;
+        const int a = (k + 1) % 3, b = (k + 2) % 3;
+        gc[a] = gc[a] + cross(c[b], q) * gD;
+        gc[b] = gc[b] + cross(q, c[a]) * gD;
+    }
+    for (int j = 0; j < 3; j++) {
+        // clip (x, y, w) <- pos: d clip_r / d pos_k = PV.c[k][r]
         for (int k = 0; k < 3; k++) {
-            const LhVec4 col = PV.c[k];
-            const float d = ((col.x * gclip.x + col.y * gclip.y) + col.z * gclip.z) + col.w * gclip.w;
+            const LhVec4& cc = PV.c[k];
+            const double d = (double(cc.x) * gc[j].x + double(cc.y) * gc[j].y) + double(cc.w) * gc[j].z;
             if (k == 0) gpos[j].x += d;
             if (k == 1) gpos[j].y += d;
             if (k == 2) gpos[j].z += d;
         }
-        const f3 gl{dot3(xyz(T.c[0]), gpos[j]), dot3(xyz(T.c[1]), gpos[j]), dot3(xyz(T.c[2]), gpos[j])};
-        add_geometry(gr, mi.positions + 12ull * sf.vi[j], gl);
+        const d3 gL{dot(col3(T.c[0]), gpos[j]), dot(col3(T.c[1]), gpos[j]), dot(col3(T.c[2]), gpos[j])};
+        add_geometry(gr, mi.positions + 12ull * sf.vi[j], gL);
     }
 }
 
 __global__ void __launch_bounds__(64) k_resolve_backward(lh_scene s, OneView vs, lh_resolve_params prm,
                                                          const float* grad_radiance, lh_resolve_grads gr,
-                                                         float* partials) {
+                                                         double* partials) {
     const lh_view& v = vs.v;
     const uint32_t tiles_x = (v.width + 7) / 8, tiles = tiles_x * ((v.height + 7) / 8);
     LightGrad lg{{0, 0, 0}, {0, 0, 0}};
     for (uint32_t t = blockIdx.x; t < tiles; t += gridDim.x) {
         const uint32_t x = (t % tiles_x) * 8 + (threadIdx.x & 7), y = (t / tiles_x) * 8 + (threadIdx.x >> 3);
         uint32_t mesh_info = ~0u;
-        f3 g_bcf{0, 0, 0};
-        float g_rf = 0.0f, g_mf = 0.0f;
+        d3 g_bcf{0, 0, 0};
+        double g_rf = 0.0, g_mf = 0.0;
         if (x < v.width && y < v.height) pixel_bwd(s, v, prm, grad_radiance, gr, x, y, mesh_info, g_bcf, g_rf, g_mf, lg);
         // material factors: one add per wave when every contributing lane shares the mesh info
         const uint64_t contrib = ballot(mesh_info != ~0u);
@@ -309,40 +385,40 @@ __global__ void __launch_bounds__(64) k_resolve_backward(lh_scene s, OneView vs,
         const uint32_t lead = __builtin_amdgcn_readlane(mesh_info, int(first_lane(contrib)));
         const bool uniform = ballot(mesh_info != ~0u && mesh_info != lead) == 0;
         if (uniform) {
-            float r[5] = {g_bcf.x, g_bcf.y, g_bcf.z, g_rf, g_mf};
+            double r[5] = {g_bcf.x, g_bcf.y, g_bcf.z, g_rf, g_mf};
             for (int i = 0; i < 5; i++) {
-                float sum = r[i];
+                double sum = r[i];
                 for (int d = 32; d >= 1; d >>= 1) sum += __shfl_xor(sum, d, 64);
                 r[i] = sum;
             }
             if (threadIdx.x == 0) {
                 if (gr.base_color_factor) {
-                    atomicAdd(gr.base_color_factor + 3 * lead, r[0]);
-                    atomicAdd(gr.base_color_factor + 3 * lead + 1, r[1]);
-                    atomicAdd(gr.base_color_factor + 3 * lead + 2, r[2]);
+                    atomicAdd(gr.base_color_factor + 3 * lead, float(r[0]));
+                    atomicAdd(gr.base_color_factor + 3 * lead + 1, float(r[1]));
+                    atomicAdd(gr.base_color_factor + 3 * lead + 2, float(r[2]));
                 }
-                if (gr.roughness_factor) atomicAdd(gr.roughness_factor + lead, r[3]);
-                if (gr.metallic_factor) atomicAdd(gr.metallic_factor + lead, r[4]);
+                if (gr.roughness_factor) atomicAdd(gr.roughness_factor + lead, float(r[3]));
+                if (gr.metallic_factor) atomicAdd(gr.metallic_factor + lead, float(r[4]));
             }
         } else if (mesh_info != ~0u) {
             if (gr.base_color_factor) {
-                atomicAdd(gr.base_color_factor + 3 * mesh_info, g_bcf.x);
-                atomicAdd(gr.base_color_factor + 3 * mesh_info + 1, g_bcf.y);
-                atomicAdd(gr.base_color_factor + 3 * mesh_info + 2, g_bcf.z);
+                atomicAdd(gr.base_color_factor + 3 * mesh_info, float(g_bcf.x));
+                atomicAdd(gr.base_color_factor + 3 * mesh_info + 1, float(g_bcf.y));
+                atomicAdd(gr.base_color_factor + 3 * mesh_info + 2, float(g_bcf.z));
             }
-            if (gr.roughness_factor) atomicAdd(gr.roughness_factor + mesh_info, g_rf);
-            if (gr.metallic_factor) atomicAdd(gr.metallic_factor + mesh_info, g_mf);
+            if (gr.roughness_factor) atomicAdd(gr.roughness_factor + mesh_info, float(g_rf));
+            if (gr.metallic_factor) atomicAdd(gr.metallic_factor + mesh_info, float(g_mf));
         }
     }
-    float r[6] = {lg.sun_dir.x, lg.sun_dir.y, lg.sun_dir.z, lg.sun_intensity.x, lg.sun_intensity.y,
-                  lg.sun_intensity.z};
+    double r[6] = {lg.sun_dir.x, lg.sun_dir.y, lg.sun_dir.z, lg.sun_intensity.x, lg.sun_intensity.y,
+                   lg.sun_intensity.z};
     for (int i = 0; i < 6; i++) {
-        float sum = r[i];
+        double sum = r[i];
         for (int d = 32; d >= 1; d >>= 1) sum += __shfl_xor(sum, d, 64);
         r[i] = sum;
     }
     if (threadIdx.x < 8) {
-        float val = 0.0f;
+        double val = 0.0;
         for (int i = 0; i < 6; i++)
             if (threadIdx.x == uint32_t(i)) val = r[i];
         partials[size_t(blockIdx.x) * 8 + threadIdx.x] = val;
@@ -351,13 +427,13 @@ __global__ void __launch_bounds__(64) k_resolve_backward(lh_scene s, OneView vs,
 
 // Sums the per-block light partials in block order (deterministic) and adds
 // them to the outputs.
-__global__ void __launch_bounds__(64) k_reduce_light(const float* partials, uint32_t blocks, lh_resolve_grads gr) {
+__global__ void __launch_bounds__(64) k_reduce_light(const double* partials, uint32_t blocks, lh_resolve_grads gr) {
     const uint32_t i = threadIdx.x;
     if (i >= 6) return;
-    float sum = 0.0f;
+    double sum = 0.0;
     for (uint32_t b = 0; b < blocks; b++) sum += partials[size_t(b) * 8 + i];
     float* dst = i < 3 ? gr.sun_dir : gr.sun_intensity;
-    if (dst) dst[i % 3] += sum;
+    if (dst) dst[i % 3] += float(sum);
 }
 
 }  // namespace
@@ -371,7 +447,7 @@ hipError_t resolve_backward(const lh_scene& s, const OneView& vs, const lh_resol
     const uint32_t blocks = std::max(1u, std::min(tiles, kBackwardBlocks));
     // light partials live in the view's raster scratch (free after the raster;
     // the C entry point checks its size against kBackwardScratchBytes)
-    float* partials = static_cast<float*>(v.scratch);
+    double* partials = static_cast<double*>(v.scratch);
     mark("resolve_backward", st);
     hipLaunchKernelGGL(k_resolve_backward, dim3(blocks), dim3(64), 0, st, s, vs, params, grad_radiance, grads,
                        partials);

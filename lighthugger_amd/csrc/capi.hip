@@ -27,6 +27,15 @@ lhd::OneView one(const lh_view* v) { return lhd::OneView{*v}; }
 
 namespace lhk {
 
+hipError_t raster_main(const lh_scene& s, const lhd::OneView& vs, const Tuning& t, hipStream_t st) {
+    hipError_t e = raster(s, vs, 0, 1, t.raster, st);
+    if (e == hipSuccess && (vs.v.flags & LH_VIEW_HIZ)) {
+        e = hiz_cull(s, vs, t.cull_blocks, st);
+        if (e == hipSuccess) e = raster(s, vs, 0, 1, t.raster, st, true);
+    }
+    return e;
+}
+
 hipError_t record_frame(const lh_scene& s, const lh_view& v, hipStream_t st) {
     const lhd::OneView vs{v};
     const uint64_t px = uint64_t(v.width) * v.height;
@@ -34,7 +43,7 @@ hipError_t record_frame(const lh_scene& s, const lh_view& v, hipStream_t st) {
     if (e == hipSuccess) e = clear_targets(vs, std::max<uint64_t>(px, uint64_t(v.shadow_size) * v.shadow_size), st);
     if (e == hipSuccess) e = cull_instances(s, vs, false, st);
     if (e == hipSuccess) e = cull_meshlets(s, vs, 0, 1, tuning_for(v).cull_blocks, st);
-    if (e == hipSuccess) e = raster(s, vs, 0, 1, tuning_for(v).raster, st);
+    if (e == hipSuccess) e = raster_main(s, vs, tuning_for(v), st);
     if (e == hipSuccess) e = depth_bounds(vs, px, st);
     if (e == hipSuccess) e = fit_cull_instances_shadow(s, vs, st);
     if (e == hipSuccess) e = cull_meshlets(s, vs, 1, LH_NUM_CASCADES, tuning_for(v).cull_blocks, st);
@@ -78,7 +87,8 @@ lh_status lh_cull_meshlets(const lh_scene* s, const lh_view* v, uint32_t pass, l
 
 lh_status lh_raster_view(const lh_scene* s, const lh_view* v, lh_stream stream) {
     if (!lhc::valid_scene(s) || !lhc::valid_view(v) || !v->visbuffer || !lhc::valid_raster(v)) return LH_ERR_INVALID;
-    return lhc::hip_status(lhk::raster(*s, one(v), 0, 1, lhk::tuning_for(*v).raster, lhc::stream_of(stream)));
+    if ((v->flags & LH_VIEW_HIZ) && !v->radiance) return LH_ERR_INVALID;  // the pyramid lives there
+    return lhc::hip_status(lhk::raster_main(*s, one(v), lhk::tuning_for(*v), lhc::stream_of(stream)));
 }
 
 lh_status lh_depth_bounds(const lh_scene* s, const lh_view* v, lh_stream stream) {

@@ -17,6 +17,7 @@
 // Every kernel is templated on its view source (one view, or a table of views
 // indexed by blockIdx.z) so a batch of views runs the same arithmetic.
 #include "cascade.h"
+#include "hiz.h"
 #include "kernels.h"
 #include "lh_device.h"
 
@@ -25,14 +26,6 @@ using namespace lhd;
 namespace {
 
 // cull_bounding_sphere (src/shaders/common/culling.glsl:1-46).
-LH_DEV void world_sphere(const Instance& in, const LhVec4& s, f3& c, float& r) {
-    c = mat4_point(in.transform, f3{s.x, s.y, s.z});
-    float sx = length3(f3{in.transform.c[0].x, in.transform.c[0].y, in.transform.c[0].z});
-    float sy = length3(f3{in.transform.c[1].x, in.transform.c[1].y, in.transform.c[1].z});
-    float sz = length3(f3{in.transform.c[2].x, in.transform.c[2].y, in.transform.c[2].z});
-    r = s.w * fmaxf(fmaxf(sx, sy), sz);
-}
-
 LH_DEV bool cull_sphere(const Instance& in, const LhVec4& s, const Uniforms& u) {
     f3 c;
     float r;
@@ -212,11 +205,15 @@ __global__ void __launch_bounds__(256) k_fit_cull_instances_shadow(lh_scene s, V
 
 // One pass per blockIdx.y (pass = pass0 + blockIdx.y), one view per
 // blockIdx.z; grid-stride over the meshlets of the pass's visible instances.
+// With LH_VIEW_HIZ (single view) the main-view survivors are split: large or
+// unbounded screen footprints go to draws[0] (phase-1 occluders), small ones
+// to the candidate list staged in draws[1].
 template <class VS>
 __global__ void __launch_bounds__(256) k_cull_meshlets(lh_scene s, VS vs, uint32_t pass0) {
     const lh_view& v = vs.get(blockIdx.z);
     const uint32_t pass = pass0 + blockIdx.y;
     const Uniforms& u = *v.uniforms;
+    const bool hiz = std::is_same<VS, OneView>::value && pass == 0 && (v.flags & LH_VIEW_HIZ) != 0;
     const uint64_t* sec = sector(v, pass);
     const uint64_t head = sec[0];
     const uint32_t total = uint32_t(head);
@@ -225,7 +222,7 @@ __global__ void __launch_bounds__(256) k_cull_meshlets(lh_scene s, VS vs, uint32
     const uint32_t stride = gridDim.x * blockDim.x;
     for (uint32_t base = blockIdx.x * blockDim.x; base < total; base += stride) {
         uint32_t id = base + threadIdx.x;
-        bool survive = false;
+        bool survive = false, candidate = false;
         PrefixSumValue e{0, 0};
         uint32_t local = 0;
         if (id < total && prefix_search(sec, n, id, e)) {
@@ -236,6 +233,14 @@ __global__ void __launch_bounds__(256) k_cull_meshlets(lh_scene s, VS vs, uint32
             bool alpha = (mi.flags & MESH_INFO_FLAGS_ALPHA_CLIP) != 0;
             if (pass == 0) {
                 survive = !cull_sphere(in, ml.bounding_sphere, u) && (alpha || !cull_cone(in, ml, u, false));
+                if (survive && hiz) {
+                    f3 c;
+                    float r;
+                    world_sphere(in, ml.bounding_sphere, c, r);
+                    const hiz::ScreenBound b = hiz::sphere_bound(u.combined_perspective_view, c, r, v.width, v.height);
+                    candidate = b.valid && max(b.x1 - b.x0, b.y1 - b.y0) <= hiz::kMaxRectPx;
+                    survive = !candidate;
+                }
             } else {
                 survive = !cull_sphere_shadow(in, ml.bounding_sphere, v.misc->shadow_view_matrices,
                                               v.misc->shadow_sphere_radii, pass - 1) &&
@@ -251,6 +256,19 @@ __global__ void __launch_bounds__(256) k_cull_meshlets(lh_scene s, VS vs, uint32
                 out[slot] = r;
             } else {
                 atomicOr(&v.counters[LH_CNT_OVERFLOW], LH_OVERFLOW_DRAWS);
+            }
+        }
+        if (hiz) {
+            const uint32_t cslot = wave_append(&v.counters[LH_CNT_HIZ_CANDIDATES], candidate);
+            if (candidate) {
+                if (cslot < v.draw_capacity) {
+                    MeshletReference r;
+                    r.instance_index = e.index;
+                    r.meshlet_index = uint16_t(local);
+                    v.draws[size_t(v.draw_capacity) + cslot] = r;
+                } else {
+                    atomicOr(&v.counters[LH_CNT_OVERFLOW], LH_OVERFLOW_DRAWS);
+                }
             }
         }
     }

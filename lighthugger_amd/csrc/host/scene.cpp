@@ -378,6 +378,37 @@ void meshlet_bounds(const MeshSource& m, const uint32_t* remap, const uint8_t* m
     out.cone_cutoff = std::sqrt(1.0f - mindp * mindp);
 }
 
+// Renumbers the vertices in first-use order over the meshlets (then any
+// unreferenced ones), so that each meshlet's vertices are nearly contiguous in
+// the position / normal / uv arrays: the raster's and the resolve's vertex
+// gathers then touch few cache lines per meshlet instead of one per vertex.
+// Geometry and meshlet contents are unchanged; only vertex numbering moves.
+void reorder_vertices_for_fetch(BuiltMesh& bm) {
+    MeshSource& m = bm.src;
+    const uint32_t nv = m.nv();
+    std::vector<uint32_t> new_of(nv, UINT32_MAX);
+    uint32_t next = 0;
+    for (uint32_t v : bm.remap)
+        if (new_of[v] == UINT32_MAX) new_of[v] = next++;
+    for (uint32_t v = 0; v < nv; v++)
+        if (new_of[v] == UINT32_MAX) new_of[v] = next++;
+    std::vector<float> pos(m.pos.size()), nrm(m.nrm.size()), uv(m.uv.size());
+    for (uint32_t v = 0; v < nv; v++) {
+        const uint32_t w = new_of[v];
+        for (int k = 0; k < 3; k++) {
+            pos[3 * w + k] = m.pos[3 * v + k];
+            nrm[3 * w + k] = m.nrm[3 * v + k];
+        }
+        uv[2 * w] = m.uv[2 * v];
+        uv[2 * w + 1] = m.uv[2 * v + 1];
+    }
+    m.pos.swap(pos);
+    m.nrm.swap(nrm);
+    m.uv.swap(uv);
+    for (uint32_t& i : m.idx) i = new_of[i];
+    for (uint32_t& r : bm.remap) r = new_of[r];
+}
+
 // Greedy meshlet builder: grows a meshlet from a seed triangle by repeatedly
 // adding the adjacent triangle that introduces the fewest new vertices, within
 // 64 vertices / 124 triangles.  The next seed is taken from the frontier of the
@@ -473,6 +504,7 @@ void build_meshlets(BuiltMesh& bm) {
     std::vector<V3> all(nv);
     for (uint32_t i = 0; i < nv; i++) all[i] = m.p(i);
     bm.sphere = bounding_sphere(all);
+    reorder_vertices_for_fetch(bm);
 }
 
 // ----------------------------------------------------------------- textures

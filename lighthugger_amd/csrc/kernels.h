@@ -62,7 +62,13 @@ hipError_t cull_meshlets(const lh_scene& s, const VS& vs, uint32_t pass0, uint32
 // view alone, or any run of cascades).
 template <class VS>
 hipError_t raster(const lh_scene& s, const VS& vs, uint32_t pass0, uint32_t npasses, const RasterTuning& t,
-                  hipStream_t st);
+                  hipStream_t st, bool phase2 = false);
+// LH_VIEW_HIZ: depth pyramid of the phase-1 raster and the occlusion test of
+// the staged candidates (appended to draws[0] for the phase-2 raster).
+hipError_t hiz_cull(const lh_scene& s, const lhd::OneView& vs, uint32_t blocks, hipStream_t st);
+// The main-view raster of one view: phase 1, and with LH_VIEW_HIZ the
+// pyramid, the occlusion test and phase 2.
+hipError_t raster_main(const lh_scene& s, const lhd::OneView& vs, const Tuning& t, hipStream_t st);
 template <class VS>
 hipError_t depth_bounds(const VS& vs, uint64_t max_pixels, hipStream_t st);
 template <class VS>
@@ -75,7 +81,7 @@ hipError_t resolve(const lh_scene& s, const VS& vs, uint32_t total_tiles, const 
 // Adjoint of the single-view resolve (lh_resolve_backward); uses the first
 // kBackwardScratchBytes of the view's scratch for per-block partial sums.
 constexpr uint32_t kBackwardBlocks = 2048;
-constexpr uint64_t kBackwardScratchBytes = uint64_t(kBackwardBlocks) * 8 * sizeof(float);
+constexpr uint64_t kBackwardScratchBytes = uint64_t(kBackwardBlocks) * 8 * sizeof(double);
 hipError_t resolve_backward(const lh_scene& s, const lhd::OneView& vs, const lh_resolve_params& params,
                             const float* grad_radiance, const lh_resolve_grads& grads, hipStream_t st);
 template <class VS>

@@ -91,6 +91,17 @@ LH_DEV LhMat4 mat4_mul(const LhMat4& a, const LhMat4& b) {
 
 LH_DEV float near_plane(const Uniforms& u) { return u.perspective.c[3].z; }
 
+// World-space bounding sphere of a mesh-space sphere under an instance
+// transform: centre transformed, radius times the largest column length
+// (src/shaders/common/culling.glsl:4-17).
+LH_DEV void world_sphere(const Instance& in, const LhVec4& s, f3& c, float& r) {
+    c = mat4_point(in.transform, f3{s.x, s.y, s.z});
+    float sx = length3(f3{in.transform.c[0].x, in.transform.c[0].y, in.transform.c[0].z});
+    float sy = length3(f3{in.transform.c[1].x, in.transform.c[1].y, in.transform.c[1].z});
+    float sz = length3(f3{in.transform.c[2].x, in.transform.c[2].y, in.transform.c[2].z});
+    r = s.w * fmaxf(fmaxf(sx, sy), sz);
+}
+
 // ---- wave64 helpers ----
 LH_DEV uint32_t lane_id() { return __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u)); }
 // Number of set bits of `mask` below this lane.

@@ -68,6 +68,7 @@ struct PassSet {
     uint32_t pass0, n;
     int32_t small_extent;   // subpixels: in-lane raster up to this vertex-bbox extent
     int32_t medium_extent;  // subpixels: wave-cooperative raster up to this extent (<= 64 px)
+    uint32_t phase2;        // main view, second occlusion phase: draws from LH_CNT_HIZ_PHASE1 on
 };
 
 LH_DEV Pass make_pass(const lh_view& v, uint32_t pass) {
@@ -313,12 +314,18 @@ struct Segments<OneView> {
     // off[i]: first flattened index of pass i; only constant indices, so the
     // array lives in registers.
     uint32_t off[LH_NUM_CASCADES + 1];
+    uint32_t first;  // draws before this index were rasterized by an earlier phase
     LH_DEV Segments(const OneView& vs, const PassSet& ps) {
         const lh_view& v = vs.v;
+        first = ps.phase2 ? min(v.counters[LH_CNT_HIZ_PHASE1], v.draw_capacity) : 0u;
         off[0] = 0;
 #pragma unroll
         for (uint32_t i = 0; i < LH_NUM_CASCADES; i++)
             off[i + 1] = off[i] + (i < ps.n ? min(v.counters[LH_CNT_DRAWS + ps.pass0 + i], v.draw_capacity) : 0u);
+        if (ps.phase2) {  // one pass: [first, count) of the main view's draws
+#pragma unroll
+            for (uint32_t i = 1; i <= LH_NUM_CASCADES; i++) off[i] = off[i] > first ? off[i] - first : 0u;
+        }
     }
     LH_DEV uint32_t total() const { return off[LH_NUM_CASCADES]; }
     LH_DEV void find(uint32_t g, uint32_t& view, uint32_t& pi, uint32_t& d) const {
@@ -332,7 +339,7 @@ struct Segments<OneView> {
                 base = off[i];
             }
         view = 0;
-        d = g - base;
+        d = g - base + first;  // first is 0 unless this is the main view's second phase
     }
 };
 
@@ -610,12 +617,12 @@ void launch_set(const lh_scene& s, const ViewTable& vs, const PassSet& ps, uint3
 
 template <class VS>
 hipError_t raster(const lh_scene& s, const VS& vs, uint32_t pass0, uint32_t npasses, const RasterTuning& t,
-                  hipStream_t st) {
+                  hipStream_t st, bool phase2) {
     if (npasses == 0 || npasses > LH_NUM_CASCADES || pass0 + npasses > LH_NUM_PASSES || (pass0 == 0 && npasses != 1))
         return hipErrorInvalidValue;
     const uint32_t small_px = std::min<uint32_t>(std::max<uint32_t>(t.small_extent_px, 1), 32);
     const uint32_t medium_px = std::min<uint32_t>(std::max(t.medium_extent_px, small_px), 64);
-    PassSet ps{pass0, npasses, int32_t(small_px * 256), int32_t(medium_px * 256)};
+    PassSet ps{pass0, npasses, int32_t(small_px * 256), int32_t(medium_px * 256), phase2 ? 1u : 0u};
     const uint32_t nv = vs.count();
     // per-view persistent grids of the queue kernels shrink with the batch size
     const uint32_t clip_blocks = nv > 1 ? std::max(t.clip_blocks / 16, 4u) : t.clip_blocks;
@@ -628,9 +635,9 @@ hipError_t raster(const lh_scene& s, const VS& vs, uint32_t pass0, uint32_t npas
 }
 
 template hipError_t raster<OneView>(const lh_scene&, const OneView&, uint32_t, uint32_t, const RasterTuning&,
-                                    hipStream_t);
+                                    hipStream_t, bool);
 template hipError_t raster<ViewTable>(const lh_scene&, const ViewTable&, uint32_t, uint32_t, const RasterTuning&,
-                                      hipStream_t);
+                                      hipStream_t, bool);
 
 hipError_t plan_segments(const ViewTable& vt, uint32_t pass0, uint32_t npasses, hipStream_t st) {
     if (vt.n * npasses > LH_MAX_BATCH_SEGMENTS) return hipErrorInvalidValue;

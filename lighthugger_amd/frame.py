@@ -142,6 +142,7 @@ class ViewConfig:
     clip_capacity: int = 1 << 18
     ldr: bool = True
     statistics: bool = False  # LH_VIEW_STATISTICS: count covered samples and depth atomics
+    hiz: bool = False  # LH_VIEW_HIZ: two-phase occlusion culling of the main view
 
 
 class ViewData:
@@ -169,7 +170,8 @@ class ViewData:
         self.c = abi.LhView(
             a["uniforms"], a["misc"], a["prefix"], a["draws"], a["counters"], a["visbuffer"], a["shadow"],
             a["radiance"], a["ldr"] if cfg.ldr else 0, a["scratch"], self.sizes["scratch"], icap, dcap, W, H, S,
-            cfg.big_capacity, cfg.clip_capacity, abi.VIEW_STATISTICS if cfg.statistics else 0,
+            cfg.big_capacity, cfg.clip_capacity,
+            (abi.VIEW_STATISTICS if cfg.statistics else 0) | (abi.VIEW_HIZ if cfg.hiz else 0),
         )
         self.uniforms = None
 

@@ -63,9 +63,10 @@ def kernel_stats(d: Path) -> list[dict]:
     for db in _files(d, "*.db"):
         con = sqlite3.connect(str(db))
         try:
+            # the database's top_kernels view reports microseconds
             cur = con.execute("SELECT name, total_calls, total_duration, average, percentage FROM top_kernels")
-            rows += [dict(name=r[0], calls=int(r[1]), total_ns=float(r[2]), avg_ns=float(r[3]), pct=float(r[4]))
-                     for r in cur]
+            rows += [dict(name=r[0], calls=int(r[1]), total_ns=float(r[2]) * 1e3, avg_ns=float(r[3]) * 1e3,
+                          pct=float(r[4])) for r in cur]
         except sqlite3.Error:
             pass
         finally:

@@ -797,9 +797,85 @@ void frozen_pixel(const lh_scene& sc, const lh_view& v, uint32_t x, uint32_t y, 
     }
 }
 
+// Conservative screen bound of a world-space sphere for the two-phase
+// occlusion test (restated from the build's device rule): the 8 corners of the
+// enclosing cube through m, pixel rectangle of their projections widened by
+// one pixel and clamped to the image, and their largest NDC depth.  Returns
+// false when a corner has w <= 1e-6 (no bound).
+bool sphere_screen_bound(const LhMat4& m, F3 c, float r, uint32_t W, uint32_t H, int32_t* rect, float& zmax) {
+    float xmin = 3.0e38f, xmax = -3.0e38f, ymin = 3.0e38f, ymax = -3.0e38f;
+    zmax = -3.0e38f;
+    const float hw = 0.5f * float(W), hh = 0.5f * float(H);
+    for (int k = 0; k < 8; k++) {
+        F3 p{c.x + ((k & 1) ? r : -r), c.y + ((k & 2) ? r : -r), c.z + ((k & 4) ? r : -r)};
+        F4 q = mat4_mul_vec4(m, F4{p.x, p.y, p.z, 1.0f});
+        if (!(q.w > 1e-6f) || !std::isfinite(q.x) || !std::isfinite(q.y) || !std::isfinite(q.z)) return false;
+        float xf = (q.x / q.w) * hw + hw, yf = (q.y / q.w) * hh + hh, z = q.z / q.w;
+        xmin = std::min(xmin, xf);
+        xmax = std::max(xmax, xf);
+        ymin = std::min(ymin, yf);
+        ymax = std::max(ymax, yf);
+        zmax = std::max(zmax, z);
+    }
+    xmin = std::max(xmin, -1.0e6f);
+    ymin = std::max(ymin, -1.0e6f);
+    xmax = std::min(xmax, 1.0e6f);
+    ymax = std::min(ymax, 1.0e6f);
+    rect[0] = std::max(int32_t(std::floor(xmin)) - 1, 0);
+    rect[1] = std::max(int32_t(std::floor(ymin)) - 1, 0);
+    rect[2] = std::min(int32_t(std::ceil(xmax)) + 1, int32_t(W) - 1);
+    rect[3] = std::min(int32_t(std::ceil(ymax)) + 1, int32_t(H) - 1);
+    return true;
+}
+
 }  // namespace
 
 extern "C" {
+
+int64_t orc_hiz_bound_violations(const lh_scene* sc, const lh_view* v, uint64_t* checked) {
+    if (!sc || !v || !checked) return -1;
+    *checked = 0;
+    const Uniforms& u = *v->uniforms;
+    const uint32_t count = std::min(v->counters[LH_CNT_DRAWS], v->draw_capacity);
+    int64_t bad = 0;
+    for (uint32_t d = 0; d < count; d++) {
+        const MeshletReference& ref = v->draws[d];
+        const Instance& in = sc->instances[ref.instance_index];
+        const MeshInfo& mi = mesh_info_of(in);
+        const Meshlet& ml = meshlet_of(mi, ref.meshlet_index);
+        F3 c;
+        float r;
+        world_sphere(in, &ml.bounding_sphere.x, c, r);
+        int32_t rect[4];
+        float zmax;
+        if (!sphere_screen_bound(u.combined_perspective_view, c, r, v->width, v->height, rect, zmax)) continue;
+        for (uint32_t k = 0; k < ml.triangle_count; k++) {
+            CV q[3];
+            bool ok = true;
+            for (uint32_t j = 0; j < 3; j++) {
+                F3 w = mat4_point(in.transform, load_position(mi, corner_vertex(mi, ml, k, j)));
+                F4 t = mat4_mul_vec4(u.combined_perspective_view, F4{w.x, w.y, w.z, 1.0f});
+                q[j] = CV{t.x, t.y, t.z, t.w};
+                ok = ok && q[j].w > 0.0f;
+            }
+            if (!ok) {
+                bad++;  // a bounded meshlet cannot have a vertex at or behind w = 0
+                continue;
+            }
+            SV s3[3];
+            for (int j = 0; j < 3; j++) project(q[j], v->width, v->height, s3[j]);
+            int32_t minX = std::min({s3[0].X, s3[1].X, s3[2].X}), maxX = std::max({s3[0].X, s3[1].X, s3[2].X});
+            int32_t minY = std::min({s3[0].Y, s3[1].Y, s3[2].Y}), maxY = std::max({s3[0].Y, s3[1].Y, s3[2].Y});
+            int32_t xlo = std::max((minX + 127) >> 8, 0), xhi = std::min((maxX - 128) >> 8, int32_t(v->width) - 1);
+            int32_t ylo = std::max((minY + 127) >> 8, 0), yhi = std::min((maxY - 128) >> 8, int32_t(v->height) - 1);
+            (*checked)++;
+            if (xlo <= xhi && ylo <= yhi && (xlo < rect[0] || xhi > rect[2] || ylo < rect[1] || yhi > rect[3])) bad++;
+            for (int j = 0; j < 3; j++)
+                if (std::min(std::max(s3[j].z, 0.0f), 1.0f) > zmax * (1.0f + 1e-4f)) bad++;
+        }
+    }
+    return bad;
+}
 
 lh_status orc_resolve_frozen(const lh_scene* sc, const lh_view* v, orc_frozen* out) {
     if (!sc || !v || !v->visbuffer || !v->uniforms || !v->misc || !out) return LH_ERR_INVALID;

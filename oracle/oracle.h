@@ -63,6 +63,13 @@ void orc_brdf(const float* v, const float* l, const float* n, float roughness, f
 /* sRGB OETF (src/shaders/display_transform.comp:19-25). */
 float orc_srgb_oetf(float x);
 
+/* Two-phase occlusion bound check: for every meshlet of draws[0], the pixel
+ * bounding box of each of its triangles (within the image) must lie in the
+ * meshlet's conservative screen rectangle and its vertex depths must not
+ * exceed the bound's depth (1e-4 relative slack).  Returns the number of
+ * violations; *checked counts the triangles examined. */
+int64_t orc_hiz_bound_violations(const lh_scene* scene, const lh_view* view, uint64_t* checked);
+
 /* ---- differentiable resolve ----
  * Per-pixel quantities the adjoint holds fixed (FP32, as the forward resolve
  * computes them): kind 0 sky, 1 shaded, 2 unsupported mesh info (16-bit

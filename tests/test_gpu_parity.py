@@ -219,3 +219,54 @@ def test_raster_statistics_match_oracle():
         assert int(g[abi.CNT_ATOMICS + p]) <= frags and (frags == 0) == (g[abi.CNT_ATOMICS + p] == 0), p
     assert g[abi.CNT_FRAGMENTS] > 10000
     assert not plain["counters"][abi.CNT_FRAGMENTS:abi.CNT_FRAGMENTS + abi.NUM_PASSES].any()
+
+
+def _hiz_pair(scene, cam, cfg_kw):
+    """The same frame with and without LH_VIEW_HIZ."""
+    import torch
+
+    from lighthugger_amd.lib import Renderer
+
+    out = []
+    for hiz in (False, True):
+        r = Renderer(scene, ViewConfig(cam.width, cam.height, hiz=hiz, **cfg_kw))
+        r.set_camera(cam)
+        r.frame()
+        torch.cuda.synchronize()
+        out.append((snapshot(r.view), r.status()))
+    return out
+
+
+@pytest.mark.parametrize("preset", ["small", "large"])
+def test_hiz_keys_identical(preset):
+    """Two-phase occlusion culling is conservative: keys, shadow depths and
+    radiance are bit-identical with it on and off, and it does cull."""
+    p = {"small": abi.PRESET_SMALL, "large": abi.PRESET_LARGE}[preset]
+    scene = HostScene.preset(p)
+    cam = preset_camera(p)
+    shadow = 1024 if preset == "small" else 2048
+    (off, st_off), (on, st_on) = _hiz_pair(scene, cam, dict(shadow_size=shadow, big_capacity=1 << 21))
+    assert st_on[abi.CNT_OVERFLOW] == 0 and st_off[abi.CNT_OVERFLOW] == 0
+    for k in ("visbuffer", "shadow", "radiance", "ldr", "misc"):
+        assert np.array_equal(off[k], on[k]), k
+    cand, phase1, drawn = int(st_on[abi.CNT_HIZ_CANDIDATES]), int(st_on[abi.CNT_HIZ_PHASE1]), int(st_on[abi.CNT_DRAWS])
+    total = int(st_off[abi.CNT_DRAWS])
+    print(f"{preset}: {total} meshlets; phase 1 {phase1}, candidates {cand}, drawn {drawn} "
+          f"({total - drawn} culled by Hi-Z)", flush=True)
+    assert phase1 + cand == total and phase1 <= drawn <= total
+    # what was drawn is a subset of the frustum/cone survivors
+    a = {tuple(x) for x in on["draws0"][["instance_index", "meshlet_index"]].tolist()}
+    b = {tuple(x) for x in off["draws0"][["instance_index", "meshlet_index"]].tolist()}
+    assert a <= b and len(a) == drawn
+    if preset == "large":
+        assert drawn < total
+
+
+def test_hiz_fixtures_identical():
+    from .fixtures import fixtures
+
+    for name, (scene, cam, cfg) in fixtures().items():
+        kw = dict(shadow_size=cfg.shadow_size, big_capacity=cfg.big_capacity, clip_capacity=cfg.clip_capacity)
+        (off, _), (on, _) = _hiz_pair(scene, cam, kw)
+        for k in ("visbuffer", "shadow", "radiance"):
+            assert np.array_equal(off[k], on[k]), (name, k)

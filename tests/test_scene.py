@@ -39,7 +39,7 @@ def test_small_preset_shape(small):
     assert len(small.mesh_infos) == 4  # one sphere x 4 materials
 
 
-def _check_meshlets(scene, mi, nverts, src_idx=None):
+def _check_meshlets(scene, mi, nverts, src=None):
     ml, remap, micro, pos = _mesh_arrays(scene, mi, nverts)
     assert (ml["index_count"] <= 64).all() and (ml["triangle_count"] <= 124).all()
     assert (ml["triangle_count"] > 0).all()
@@ -56,13 +56,20 @@ def _check_meshlets(scene, mi, nverts, src_idx=None):
         d = np.linalg.norm(p[np.unique(c)] - cs, axis=1)
         assert (d <= r * (1 + 1e-5) + 1e-6).all(), "meshlet sphere must contain its vertices"
     tris = np.concatenate(tris)
-    if src_idx is not None:
-        key = lambda t: np.sort(t, axis=1)  # noqa: E731
-        a = key(tris)
-        b = key(src_idx)
-        a = a[np.lexsort(a.T[::-1])]
-        b = b[np.lexsort(b.T[::-1])]
-        assert np.array_equal(a, b), "meshlets must hold every source triangle exactly once"
+    if src is not None:
+        # compared as position triples: the producer renumbers vertices
+        # (meshlet first-use order) but keeps every triangle exactly once
+        src_pos, src_idx = np.asarray(src[0], np.float32), np.asarray(src[1])
+
+        def key(P, idx):
+            t = P[idx.astype(np.int64)].reshape(len(idx), 9)
+            t = t.reshape(-1, 3, 3)
+            t = np.sort(t.view([("x", "f4"), ("y", "f4"), ("z", "f4")]).reshape(-1, 3), axis=1)
+            t = t.view(np.float32).reshape(-1, 9)
+            return t[np.lexsort(t.T[::-1])]
+
+        assert np.array_equal(key(pos, tris), key(src_pos, src_idx)), \
+            "meshlets must hold every source triangle exactly once"
     sph = scene.mesh_infos[mi]["bounding_sphere"]
     d = np.linalg.norm(pos - sph[:3], axis=1)
     assert (d <= sph[3] * (1 + 1e-5) + 1e-6).all()
@@ -77,7 +84,7 @@ def test_small_meshlets(small):
 def test_grid_and_cube_meshlets_cover_every_triangle():
     for mesh in (grid_patch(20, 17), cube_mesh()):
         s = single_scene(mesh, [np.eye(4, dtype=np.float32)])
-        _check_meshlets(s, 0, len(mesh[0]), mesh[3])
+        _check_meshlets(s, 0, len(mesh[0]), (mesh[0], mesh[3]))
 
 
 def test_mesh_at_exactly_64_vertices_and_124_triangles():
@@ -87,7 +94,7 @@ def test_mesh_at_exactly_64_vertices_and_124_triangles():
     mesh = grid_patch(7, 7)
     assert len(mesh[0]) == 64 and len(mesh[3]) == 98
     s = single_scene(mesh, [np.eye(4, dtype=np.float32)])
-    ml = _check_meshlets(s, 0, 64, mesh[3])
+    ml = _check_meshlets(s, 0, 64, (mesh[0], mesh[3]))
     assert len(ml) == 1 and ml[0]["index_count"] == 64 and ml[0]["triangle_count"] == 98
 
 
