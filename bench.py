@@ -44,8 +44,11 @@ def parse():
     ap.add_argument("--copy-probe", action="store_true",
                     help="run 256 MiB device-to-device copies (HBM-counter calibration for rocprofv3 --pmc)")
     ap.add_argument("--no-atomic-probe", action="store_true", help="skip the atomicMin ceiling measurement")
-    ap.add_argument("--no-hiz", action="store_true",
-                    help="time the frame without two-phase occlusion culling (output is identical either way)")
+    ap.add_argument("--hiz", action="store_true",
+                    help="time the frame with two-phase occlusion culling (output is identical either way; the "
+                         "other setting is timed too)")
+    ap.add_argument("--diff-only", action="store_true",
+                    help="only the differentiable-resolve measurement (JSON), e.g. under a profiler")
     ap.add_argument("--batch-profile", action="store_true",
                     help="per-kernel device times of the 64-camera batch, per-view loop vs batched (JSON)")
     return ap.parse_args()
@@ -329,6 +332,9 @@ def main():
     from lighthugger_amd.scene import HostScene, make_uniforms, preset_camera
 
     preset = {"large": abi.PRESET_LARGE, "small": abi.PRESET_SMALL, "stress": abi.PRESET_STRESS}[args.workload]
+    if args.diff_only:
+        print(json.dumps(diff_bench(max(1, args.diff_steps))), flush=True)
+        return
     if args.batch_profile:
         print(json.dumps(batch_profile(HostScene.preset(preset))), flush=True)
         return
@@ -336,7 +342,7 @@ def main():
     t0 = time.time()
     scene = HostScene.preset(preset)
     cam = preset_camera(preset)
-    cfg = ViewConfig(cam.width, cam.height, shadow, big_capacity=1 << 21, clip_capacity=1 << 18, hiz=not args.no_hiz)
+    cfg = ViewConfig(cam.width, cam.height, shadow, big_capacity=1 << 21, clip_capacity=1 << 18, hiz=args.hiz)
     r = Renderer(scene, cfg)
     r.set_camera(cam)
     print(f"# scene built in {time.time() - t0:.1f}s: {scene.num_instances} instances, "

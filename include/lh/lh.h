@@ -54,7 +54,7 @@ typedef void* lh_stream; /* hipStream_t */
 #define LH_CNT_HIZ_CANDIDATES 26 /* LH_VIEW_HIZ: small meshlets tested against the depth pyramid */
 #define LH_CNT_HIZ_PHASE1 27     /* LH_VIEW_HIZ: main-view draws rasterized before the pyramid */
 #define LH_CNT_FRAGMENTS 32 /* + pass: covered samples (only with LH_VIEW_STATISTICS) */
-#define LH_CNT_ATOMICS 40   /* + pass: depth atomics issued after the early-out test (same) */
+#define LH_CNT_ATOMICS 40   /* + pass: depth atomics issued (same; main view: samples with depth > 0) */
 #define LH_NUM_COUNTERS 64
 #define LH_OVERFLOW_INSTANCES 0x1u
 #define LH_OVERFLOW_DRAWS 0x2u
@@ -66,11 +66,13 @@ typedef void* lh_stream; /* hipStream_t */
 #define LH_VIEW_STATISTICS 0x1u /* single-view raster also counts LH_CNT_FRAGMENTS / LH_CNT_ATOMICS
                                    (slower; batched launches ignore it) */
 #define LH_VIEW_HIZ 0x2u        /* two-phase occlusion culling of the main view (single view only):
-                                   meshlets with a small screen footprint are tested against a depth
-                                   pyramid of what the large ones left and skipped when hidden.  The
-                                   final keys, shadows and radiance are those of the frame without it;
-                                   draws[0] then lists only the meshlets actually rasterized, and the
-                                   candidates are staged in draws[1] (free until the shadow passes). */
+                                   phase 1 rasterizes the meshlets with a large screen footprint and
+                                   those visible in this view's last frame (lh_view.history); the rest
+                                   are tested against a depth pyramid of phase 1 and skipped when
+                                   hidden.  The final keys, shadows and radiance are those of the frame
+                                   without it; draws[0] then lists only the meshlets actually
+                                   rasterized, and the candidates are staged in draws[1] (free until
+                                   the shadow passes). */
 
 /* Bytes per entry of the two raster queues inside lh_view.scratch:
  * [clip queue: clip_capacity x 8 B][pad to 256][big queue: big_capacity x 48 B]. */
@@ -111,6 +113,9 @@ typedef struct lh_view {
     uint32_t big_capacity;
     uint32_t clip_capacity;
     uint32_t flags;             /* LH_VIEW_* bits */
+    uint32_t* history;          /* LH_VIEW_HIZ: one bit per stable meshlet id (total_meshlets bits),
+                                   set for the meshlets that own a pixel of the last frame of this
+                                   view; may be NULL.  Zero it when the view is created. */
 } lh_view;
 
 /* Bytes of scratch needed for the given queue capacities. */

@@ -138,7 +138,7 @@ class LhView(C.Structure):
         ("radiance", C.c_uint64), ("ldr", C.c_uint64), ("scratch", C.c_uint64), ("scratch_bytes", C.c_uint64),
         ("instance_capacity", C.c_uint32), ("draw_capacity", C.c_uint32), ("width", C.c_uint32),
         ("height", C.c_uint32), ("shadow_size", C.c_uint32), ("big_capacity", C.c_uint32),
-        ("clip_capacity", C.c_uint32), ("flags", C.c_uint32),
+        ("clip_capacity", C.c_uint32), ("flags", C.c_uint32), ("history", C.c_uint64),
     ]
 
 

@@ -15,11 +15,11 @@
 // lambda_j = D_j / (D_0 + D_1 + D_2), D_j = q . (c_{j+1} x c_{j+2}).
 //
 // Work split: a persistent grid of one-wave blocks walks the 8x8 pixel tiles.
-// Per pixel the gradients of its three vertices go to the gradient arena with
-// float atomics; material-factor gradients are summed over the wave when the
-// wave sees one mesh info (the common case) and added once; the six light
-// gradients stay in registers and every block writes one partial, which a
-// second one-block kernel sums in a fixed order.
+// Per tile the gradients of the pixels' vertices are merged in an LDS hash
+// table and added to the gradient arena once per distinct vec3; material-factor
+// gradients are summed over the wave per distinct mesh info and added once;
+// the six light gradients stay in registers and every block writes one
+// partial, which a second one-block kernel sums in a fixed order.
 #include "kernels.h"
 #include "shading.h"
 
@@ -192,13 +192,48 @@ struct LightGrad {
     d3 sun_dir, sun_intensity;
 };
 
-LH_DEV void add_geometry(const lh_resolve_grads& gr, uint64_t addr, d3 g) {
+// Vertex gradients of one tile are merged in an LDS hash table keyed by the
+// gradient-arena word of the vec3 (positions and normals alike) and flushed
+// with one atomic per distinct word: neighbouring pixels share vertices, and
+// same-address atomics from one wave serialize.  A tile inserts at most
+// 64 pixels x 6 vec3 = 384 keys, so the 512-slot table never fills.
+constexpr uint32_t kHashSlots = 512;
+constexpr uint32_t kHashEmpty = 0xFFFFFFFFu;
+struct VertexHash {
+    uint32_t* keys;  // [kHashSlots]
+    float* vals;     // [kHashSlots * 3]
+};
+
+LH_DEV void add_geometry(const lh_resolve_grads& gr, const VertexHash& h, uint64_t addr, d3 g) {
     if (!gr.geometry || addr < gr.geometry_base) return;
-    const uint64_t k = (addr - gr.geometry_base) / 4;
-    if (k + 3 > gr.geometry_bytes / 4) return;
-    atomicAdd(gr.geometry + k, float(g.x));
-    atomicAdd(gr.geometry + k + 1, float(g.y));
-    atomicAdd(gr.geometry + k + 2, float(g.z));
+    const uint64_t k64 = (addr - gr.geometry_base) / 4;
+    if (k64 + 3 > gr.geometry_bytes / 4 || k64 >= kHashEmpty) return;
+    const uint32_t k = uint32_t(k64);
+    uint32_t slot = (k * 2654435761u) >> 23;  // 9 bits: kHashSlots == 512
+    while (true) {
+        const uint32_t old = atomicCAS(h.keys + slot, kHashEmpty, k);
+        if (old == kHashEmpty || old == k) break;
+        slot = (slot + 1) & (kHashSlots - 1);
+    }
+    atomicAdd(h.vals + 3 * slot, float(g.x));
+    atomicAdd(h.vals + 3 * slot + 1, float(g.y));
+    atomicAdd(h.vals + 3 * slot + 2, float(g.z));
+}
+
+// Called by the whole wave after a tile: adds every merged vec3 to the arena
+// and empties the table.
+LH_DEV void flush_geometry(const lh_resolve_grads& gr, const VertexHash& h) {
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < kHashSlots; i += 64) {
+        const uint32_t k = h.keys[i];
+        if (k == kHashEmpty) continue;
+        atomicAdd(gr.geometry + k, h.vals[3 * i]);
+        atomicAdd(gr.geometry + k + 1, h.vals[3 * i + 1]);
+        atomicAdd(gr.geometry + k + 2, h.vals[3 * i + 2]);
+        h.keys[i] = kHashEmpty;
+        h.vals[3 * i] = h.vals[3 * i + 1] = h.vals[3 * i + 2] = 0.0f;
+    }
+    __syncthreads();
 }
 
 LH_DEV bool supported(const MeshInfo& mi) {
@@ -240,8 +275,8 @@ LH_DEV Frozen frozen(const lh_scene& s, const lh_view& v, const MeshInfo& mi, co
 // light gradients to lg.  The function and its reverse are evaluated in FP64
 // from the FP32 inputs.
 LH_DEV void pixel_bwd(const lh_scene& s, const lh_view& v, const lh_resolve_params& prm, const float* grad_radiance,
-                      const lh_resolve_grads& gr, uint32_t x, uint32_t y, uint32_t& mesh_info, d3& g_bcf, double& g_rf,
-                      double& g_mf, LightGrad& lg) {
+                      const lh_resolve_grads& gr, const VertexHash& hash, uint32_t x, uint32_t y, uint32_t& mesh_info,
+                      d3& g_bcf, double& g_rf, double& g_mf, LightGrad& lg) {
     mesh_info = ~0u;
     const size_t o = size_t(y) * v.width + x;
     Surface sf;
@@ -343,7 +378,7 @@ LH_DEV void pixel_bwd(const lh_scene& s, const lh_view& v, const lh_resolve_para
         glam[j] = (dot(gworld, pos[j] - pos[0]) + dot(gnI, nn[j])) +
                   (guv.x * (uvs[j].x - uvs[0].x) + guv.y * (uvs[j].y - uvs[0].y));
         gpos[j] = gworld * lam[j];
-        add_geometry(gr, mi.normals + 12ull * sf.vi[j], normalize_bwd(nraw[j], gnI * lam[j]));
+        add_geometry(gr, hash, mi.normals + 12ull * sf.vi[j], normalize_bwd(nraw[j], gnI * lam[j]));
     }
     const double lg_dot = (glam[0] * lam[0] + glam[1] * lam[1]) + glam[2] * lam[2];
     d3 gc[3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
@@ -363,13 +398,21 @@ LH_DEV void pixel_bwd(const lh_scene& s, const lh_view& v, const lh_resolve_para
             if (k == 2) gpos[j].z += d;
         }
         const d3 gL{dot(col3(T.c[0]), gpos[j]), dot(col3(T.c[1]), gpos[j]), dot(col3(T.c[2]), gpos[j])};
-        add_geometry(gr, mi.positions + 12ull * sf.vi[j], gL);
+        add_geometry(gr, hash, mi.positions + 12ull * sf.vi[j], gL);
     }
 }
 
 __global__ void __launch_bounds__(64) k_resolve_backward(lh_scene s, OneView vs, lh_resolve_params prm,
                                                          const float* grad_radiance, lh_resolve_grads gr,
                                                          double* partials) {
+    __shared__ uint32_t hkeys[kHashSlots];
+    __shared__ float hvals[kHashSlots * 3];
+    const VertexHash hash{hkeys, hvals};
+    for (uint32_t i = threadIdx.x; i < kHashSlots; i += 64) {
+        hkeys[i] = kHashEmpty;
+        hvals[3 * i] = hvals[3 * i + 1] = hvals[3 * i + 2] = 0.0f;
+    }
+    __syncthreads();
     const lh_view& v = vs.v;
     const uint32_t tiles_x = (v.width + 7) / 8, tiles = tiles_x * ((v.height + 7) / 8);
     LightGrad lg{{0, 0, 0}, {0, 0, 0}};
@@ -378,20 +421,22 @@ __global__ void __launch_bounds__(64) k_resolve_backward(lh_scene s, OneView vs,
         uint32_t mesh_info = ~0u;
         d3 g_bcf{0, 0, 0};
         double g_rf = 0.0, g_mf = 0.0;
-        if (x < v.width && y < v.height) pixel_bwd(s, v, prm, grad_radiance, gr, x, y, mesh_info, g_bcf, g_rf, g_mf, lg);
-        // material factors: one add per wave when every contributing lane shares the mesh info
-        const uint64_t contrib = ballot(mesh_info != ~0u);
-        if (contrib == 0) continue;
-        const uint32_t lead = __builtin_amdgcn_readlane(mesh_info, int(first_lane(contrib)));
-        const bool uniform = ballot(mesh_info != ~0u && mesh_info != lead) == 0;
-        if (uniform) {
-            double r[5] = {g_bcf.x, g_bcf.y, g_bcf.z, g_rf, g_mf};
+        if (x < v.width && y < v.height)
+            pixel_bwd(s, v, prm, grad_radiance, gr, hash, x, y, mesh_info, g_bcf, g_rf, g_mf, lg);
+        if (gr.geometry) flush_geometry(gr, hash);
+        // material factors: one add per distinct mesh info of the wave
+        uint64_t pending = ballot(mesh_info != ~0u);
+        while (pending) {
+            const uint32_t lead = __builtin_amdgcn_readlane(mesh_info, int(first_lane(pending)));
+            const bool mine = mesh_info == lead;
+            double r[5] = {mine ? g_bcf.x : 0.0, mine ? g_bcf.y : 0.0, mine ? g_bcf.z : 0.0, mine ? g_rf : 0.0,
+                           mine ? g_mf : 0.0};
             for (int i = 0; i < 5; i++) {
                 double sum = r[i];
                 for (int d = 32; d >= 1; d >>= 1) sum += __shfl_xor(sum, d, 64);
                 r[i] = sum;
             }
-            if (threadIdx.x == 0) {
+            if (lane_id() == first_lane(pending)) {
                 if (gr.base_color_factor) {
                     atomicAdd(gr.base_color_factor + 3 * lead, float(r[0]));
                     atomicAdd(gr.base_color_factor + 3 * lead + 1, float(r[1]));
@@ -400,14 +445,7 @@ __global__ void __launch_bounds__(64) k_resolve_backward(lh_scene s, OneView vs,
                 if (gr.roughness_factor) atomicAdd(gr.roughness_factor + lead, float(r[3]));
                 if (gr.metallic_factor) atomicAdd(gr.metallic_factor + lead, float(r[4]));
             }
-        } else if (mesh_info != ~0u) {
-            if (gr.base_color_factor) {
-                atomicAdd(gr.base_color_factor + 3 * mesh_info, float(g_bcf.x));
-                atomicAdd(gr.base_color_factor + 3 * mesh_info + 1, float(g_bcf.y));
-                atomicAdd(gr.base_color_factor + 3 * mesh_info + 2, float(g_bcf.z));
-            }
-            if (gr.roughness_factor) atomicAdd(gr.roughness_factor + mesh_info, float(g_rf));
-            if (gr.metallic_factor) atomicAdd(gr.metallic_factor + mesh_info, float(g_mf));
+            pending &= ~ballot(mine);
         }
     }
     double r[6] = {lg.sun_dir.x, lg.sun_dir.y, lg.sun_dir.z, lg.sun_intensity.x, lg.sun_intensity.y,

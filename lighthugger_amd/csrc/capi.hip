@@ -32,6 +32,7 @@ hipError_t raster_main(const lh_scene& s, const lhd::OneView& vs, const Tuning& 
     if (e == hipSuccess && (vs.v.flags & LH_VIEW_HIZ)) {
         e = hiz_cull(s, vs, t.cull_blocks, st);
         if (e == hipSuccess) e = raster(s, vs, 0, 1, t.raster, st, true);
+        if (e == hipSuccess) e = update_history(s, vs, st);
     }
     return e;
 }

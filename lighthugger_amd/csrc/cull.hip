@@ -234,12 +234,18 @@ __global__ void __launch_bounds__(256) k_cull_meshlets(lh_scene s, VS vs, uint32
             if (pass == 0) {
                 survive = !cull_sphere(in, ml.bounding_sphere, u) && (alpha || !cull_cone(in, ml, u, false));
                 if (survive && hiz) {
-                    f3 c;
-                    float r;
-                    world_sphere(in, ml.bounding_sphere, c, r);
-                    const hiz::ScreenBound b = hiz::sphere_bound(u.combined_perspective_view, c, r, v.width, v.height);
-                    candidate = b.valid && max(b.x1 - b.x0, b.y1 - b.y0) <= hiz::kMaxRectPx;
-                    survive = !candidate;
+                    // phase 1: visible in this view's last frame, or a large footprint
+                    const uint32_t mid = s.instance_meshlet_base[e.index] + local;
+                    const bool seen = v.history && ((v.history[mid >> 5] >> (mid & 31)) & 1u);
+                    if (!seen) {
+                        f3 c;
+                        float r;
+                        world_sphere(in, ml.bounding_sphere, c, r);
+                        const hiz::ScreenBound b =
+                            hiz::sphere_bound(u.combined_perspective_view, c, r, v.width, v.height);
+                        candidate = b.valid && max(b.x1 - b.x0, b.y1 - b.y0) <= hiz::kMaxRectPx;
+                        survive = !candidate;
+                    }
                 }
             } else {
                 survive = !cull_sphere_shadow(in, ml.bounding_sphere, v.misc->shadow_view_matrices,

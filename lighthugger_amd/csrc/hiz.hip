@@ -76,9 +76,57 @@ __global__ void __launch_bounds__(256) k_hiz_cull(lh_scene s, OneView vs) {
     }
 }
 
+// Visibility history of the frame just rasterized: clear, then one bit per
+// meshlet that owns a pixel.  Lanes of a wave that hit the same 32-bit word
+// merge their bits first, so each word takes one atomic per wave.
+__global__ void __launch_bounds__(256) k_history_clear(OneView vs, uint32_t words) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < words; i += gridDim.x * blockDim.x)
+        vs.v.history[i] = 0u;
+}
+
+__global__ void __launch_bounds__(256) k_history_mark(OneView vs, uint32_t total_meshlets) {
+    const lh_view& v = vs.v;
+    const uint64_t n = uint64_t(v.width) * v.height;
+    for (uint64_t base = uint64_t(blockIdx.x) * blockDim.x; base < n; base += uint64_t(gridDim.x) * blockDim.x) {
+        const uint64_t i = base + threadIdx.x;
+        uint32_t word = ~0u, bit = 0u;
+        if (i < n) {
+            const uint64_t key = v.visbuffer[i];
+            const uint32_t mid = uint32_t(key >> LH_KEY_TRI_BITS) & (LH_MAX_SCENE_MESHLETS - 1);
+            if (key != LH_VISBUFFER_CLEAR && mid < total_meshlets) {
+                word = mid >> 5;
+                bit = 1u << (mid & 31);
+            }
+        }
+        uint64_t pending = ballot(word != ~0u);
+        while (pending) {
+            const uint32_t w = __builtin_amdgcn_readlane(word, int(first_lane(pending)));
+            const bool mine = word == w;
+            uint32_t bits = mine ? bit : 0u;
+            for (int d = 32; d >= 1; d >>= 1) bits |= __shfl_xor(bits, d, 64);
+            if (lane_id() == first_lane(pending)) atomicOr(v.history + w, bits);
+            pending &= ~ballot(mine);
+        }
+    }
+}
+
 }  // namespace
 
 namespace lhk {
+
+hipError_t update_history(const lh_scene& s, const OneView& vs, hipStream_t st) {
+    const lh_view& v = vs.v;
+    if (!v.history) return hipSuccess;
+    const uint32_t words = (s.total_meshlets + 31) / 32;
+    const uint64_t px = uint64_t(v.width) * v.height;
+    mark("history_update", st);
+    hipLaunchKernelGGL(k_history_clear, dim3(std::max(1u, std::min((words + 255) / 256, 256u))), dim3(256), 0, st,
+                       vs, words);
+    hipLaunchKernelGGL(k_history_mark, dim3(uint32_t(std::min<uint64_t>((px + 255) / 256, 2048))), dim3(256), 0, st,
+                       vs, s.total_meshlets);
+    return hipGetLastError();
+}
+
 
 hipError_t hiz_cull(const lh_scene& s, const OneView& vs, uint32_t blocks, hipStream_t st) {
     const lh_view& v = vs.v;

@@ -66,6 +66,8 @@ hipError_t raster(const lh_scene& s, const VS& vs, uint32_t pass0, uint32_t npas
 // LH_VIEW_HIZ: depth pyramid of the phase-1 raster and the occlusion test of
 // the staged candidates (appended to draws[0] for the phase-2 raster).
 hipError_t hiz_cull(const lh_scene& s, const lhd::OneView& vs, uint32_t blocks, hipStream_t st);
+// LH_VIEW_HIZ: rewrites the view's visibility history from its visibility buffer.
+hipError_t update_history(const lh_scene& s, const lhd::OneView& vs, hipStream_t st);
 // The main-view raster of one view: phase 1, and with LH_VIEW_HIZ the
 // pyramid, the occlusion test and phase 2.
 hipError_t raster_main(const lh_scene& s, const lhd::OneView& vs, const Tuning& t, hipStream_t st);

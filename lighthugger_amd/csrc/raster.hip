@@ -188,6 +188,9 @@ struct Tally<true> {
 };
 
 // Depth test of one covered sample; returns whether an atomic was issued.
+// The atomic is issued without reading the target first: a read-before-
+// atomic early-out saves atomics but makes every sample wait for a load,
+// and measured slower (the raster is latency-bound, not atomic-bound).
 template <int KIND>
 LH_DEV bool fragment(const Pass& p, int32_t px, int32_t py, float z, uint32_t id) {
     z = fminf(fmaxf(z, 0.0f), 1.0f);
@@ -195,13 +198,9 @@ LH_DEV bool fragment(const Pass& p, int32_t px, int32_t py, float z, uint32_t id
     if (KIND == 0) {
         if (!(z > 0.0f)) return false;  // depth GREATER against the 0.0 clear
         unsigned long long key = ((unsigned long long)(~__float_as_uint(z)) << 32) | id;
-        unsigned long long* a = reinterpret_cast<unsigned long long*>(p.vis + o);
-        if (!(key < *a)) return false;  // a stale value is never smaller: safe early-out
-        atomicMin(a, key);
+        atomicMin(reinterpret_cast<unsigned long long*>(p.vis + o), key);
     } else {
-        uint32_t d = __float_as_uint(z);
-        if (!(d < p.depth[o])) return false;
-        atomicMin(p.depth + o, d);
+        atomicMin(p.depth + o, __float_as_uint(z));
     }
     return true;
 }

@@ -159,6 +159,7 @@ class ViewData:
             ("draws", abi.NUM_PASSES * dcap * 8), ("counters", abi.NUM_COUNTERS * 4), ("visbuffer", W * H * 8),
             ("shadow", abi.NUM_CASCADES * S * S * 4), ("radiance", W * H * 16), ("ldr", W * H * 4 if cfg.ldr else 0),
             ("scratch", abi.scratch_bytes(cfg.big_capacity, cfg.clip_capacity)),
+            ("history", (scene.scene.total_meshlets + 31) // 32 * 4),
         ]
         self.offsets, self.sizes, total = {}, {}, 0
         for name, nb in layout:
@@ -171,8 +172,9 @@ class ViewData:
             a["uniforms"], a["misc"], a["prefix"], a["draws"], a["counters"], a["visbuffer"], a["shadow"],
             a["radiance"], a["ldr"] if cfg.ldr else 0, a["scratch"], self.sizes["scratch"], icap, dcap, W, H, S,
             cfg.big_capacity, cfg.clip_capacity,
-            (abi.VIEW_STATISTICS if cfg.statistics else 0) | (abi.VIEW_HIZ if cfg.hiz else 0),
+            (abi.VIEW_STATISTICS if cfg.statistics else 0) | (abi.VIEW_HIZ if cfg.hiz else 0), a["history"],
         )
+        mem.write(self.handle, self.offsets["history"], np.zeros(self.sizes["history"], np.uint8))
         self.uniforms = None
 
     @property

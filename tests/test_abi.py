@@ -32,7 +32,7 @@ int main(void) {
   O(MeshInfo, base_color_factor) O(Instance, normal_transform) O(Meshlet, index_count)
   O(Uniforms, debug_shadowmaps) O(MiscStorage, max_depth) O(LhTexture, level_offset)
   O(lh_scene, num_instances) O(lh_scene, num_textures) O(lh_view, scratch_bytes) O(lh_view, instance_capacity)
-  O(lh_view, flags) O(lh_camera, shadow_cam_distance) O(lh_scene_arrays, unique_triangles)
+  O(lh_view, flags) O(lh_view, history) O(lh_camera, shadow_cam_distance) O(lh_scene_arrays, unique_triangles)
   return 0;
 }
 """
@@ -77,6 +77,7 @@ def test_field_offsets(probe):
     assert probe["lh_view.scratch_bytes"] == abi.LhView.scratch_bytes.offset
     assert probe["lh_view.instance_capacity"] == abi.LhView.instance_capacity.offset
     assert probe["lh_view.flags"] == abi.LhView.flags.offset
+    assert probe["lh_view.history"] == abi.LhView.history.offset
     assert probe["lh_camera.shadow_cam_distance"] == abi.Camera.shadow_cam_distance.offset
     assert probe["lh_scene_arrays.unique_triangles"] == abi.SceneArrays.unique_triangles.offset
 

@@ -234,6 +234,10 @@ def _hiz_pair(scene, cam, cfg_kw):
         r.frame()
         torch.cuda.synchronize()
         out.append((snapshot(r.view), r.status()))
+    # a second occlusion-culled frame: phase 1 starts from the first frame's visibility
+    r.frame()
+    torch.cuda.synchronize()
+    out.append((snapshot(r.view), r.status()))
     return out
 
 
@@ -245,10 +249,13 @@ def test_hiz_keys_identical(preset):
     scene = HostScene.preset(p)
     cam = preset_camera(p)
     shadow = 1024 if preset == "small" else 2048
-    (off, st_off), (on, st_on) = _hiz_pair(scene, cam, dict(shadow_size=shadow, big_capacity=1 << 21))
-    assert st_on[abi.CNT_OVERFLOW] == 0 and st_off[abi.CNT_OVERFLOW] == 0
+    (off, st_off), (on, st_on), (warm, st_warm) = _hiz_pair(scene, cam, dict(shadow_size=shadow, big_capacity=1 << 21))
+    assert st_on[abi.CNT_OVERFLOW] == 0 and st_off[abi.CNT_OVERFLOW] == 0 and st_warm[abi.CNT_OVERFLOW] == 0
     for k in ("visbuffer", "shadow", "radiance", "ldr", "misc"):
         assert np.array_equal(off[k], on[k]), k
+        assert np.array_equal(off[k], warm[k]), k
+    print(f"{preset} second frame: phase 1 {int(st_warm[abi.CNT_HIZ_PHASE1])}, candidates "
+          f"{int(st_warm[abi.CNT_HIZ_CANDIDATES])}, drawn {int(st_warm[abi.CNT_DRAWS])}", flush=True)
     cand, phase1, drawn = int(st_on[abi.CNT_HIZ_CANDIDATES]), int(st_on[abi.CNT_HIZ_PHASE1]), int(st_on[abi.CNT_DRAWS])
     total = int(st_off[abi.CNT_DRAWS])
     print(f"{preset}: {total} meshlets; phase 1 {phase1}, candidates {cand}, drawn {drawn} "
@@ -267,6 +274,7 @@ def test_hiz_fixtures_identical():
 
     for name, (scene, cam, cfg) in fixtures().items():
         kw = dict(shadow_size=cfg.shadow_size, big_capacity=cfg.big_capacity, clip_capacity=cfg.clip_capacity)
-        (off, _), (on, _) = _hiz_pair(scene, cam, kw)
+        (off, _), (on, _), (warm, _) = _hiz_pair(scene, cam, kw)
         for k in ("visbuffer", "shadow", "radiance"):
             assert np.array_equal(off[k], on[k]), (name, k)
+            assert np.array_equal(off[k], warm[k]), (name, k)
