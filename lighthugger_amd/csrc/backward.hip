@@ -402,7 +402,18 @@ LH_DEV void pixel_bwd(const lh_scene& s, const lh_view& v, const lh_resolve_para
     }
 }
 
-__global__ void __launch_bounds__(64) k_resolve_backward(lh_scene s, OneView vs, lh_resolve_params prm,
+// Occupancy target (waves per SIMD) of the adjoint; 0 leaves the register
+// budget to the compiler.
+#ifndef LH_BACKWARD_WAVES
+#define LH_BACKWARD_WAVES 0
+#endif
+#if LH_BACKWARD_WAVES > 0
+#define LH_BACKWARD_OCCUPANCY LH_WAVES_ATTR(LH_BACKWARD_WAVES)
+#else
+#define LH_BACKWARD_OCCUPANCY
+#endif
+
+__global__ void __launch_bounds__(64) LH_BACKWARD_OCCUPANCY k_resolve_backward(lh_scene s, OneView vs, lh_resolve_params prm,
                                                          const float* grad_radiance, lh_resolve_grads gr,
                                                          double* partials) {
     __shared__ uint32_t hkeys[kHashSlots];

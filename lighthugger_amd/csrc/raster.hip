@@ -327,6 +327,10 @@ struct Segments<OneView> {
         }
     }
     LH_DEV uint32_t total() const { return off[LH_NUM_CASCADES]; }
+    static constexpr uint32_t kChunk = 1;  // lookups are register-only: plain grid stride
+    struct Cursor {};
+    LH_DEV Cursor cursor(uint32_t) const { return {}; }
+    LH_DEV void at(Cursor&, uint32_t g, uint32_t& view, uint32_t& pi, uint32_t& d) const { find(g, view, pi, d); }
     LH_DEV void find(uint32_t g, uint32_t& view, uint32_t& pi, uint32_t& d) const {
         // the last pass whose range starts at or before g (empty passes are skipped)
         uint32_t base = 0;
@@ -348,7 +352,13 @@ struct Segments<ViewTable> {
     uint32_t nseg, np;
     LH_DEV Segments(const ViewTable& vs, const PassSet& ps) : off(vs.seg_offsets), nseg(vs.n * ps.n), np(ps.n) {}
     LH_DEV uint32_t total() const { return off[nseg]; }
-    LH_DEV void find(uint32_t g, uint32_t& view, uint32_t& pi, uint32_t& d) const {
+    // A block takes chunks of kChunk consecutive draws: one binary search over
+    // the segment table per chunk, then the cursor walks forward.
+    static constexpr uint32_t kChunk = 8;
+    struct Cursor {
+        uint32_t k, begin, end;  // segment k covers draws [begin, end)
+    };
+    LH_DEV Cursor cursor(uint32_t g) const {
         uint32_t lo = 0, hi = nseg;  // last k with off[k] <= g
         while (hi - lo > 1) {
             uint32_t mid = (lo + hi) / 2;
@@ -357,9 +367,18 @@ struct Segments<ViewTable> {
             else
                 hi = mid;
         }
-        view = lo / np;
-        pi = lo % np;
-        d = g - off[lo];
+        return {lo, off[lo], off[lo + 1]};
+    }
+    // g < total() and not below the cursor's segment
+    LH_DEV void at(Cursor& c, uint32_t g, uint32_t& view, uint32_t& pi, uint32_t& d) const {
+        while (g >= c.end) {
+            c.k++;
+            c.begin = c.end;
+            c.end = off[c.k + 1];
+        }
+        view = c.k / np;
+        pi = c.k % np;
+        d = g - c.begin;
     }
 };
 
@@ -372,9 +391,7 @@ __global__ void __launch_bounds__(64) LH_RASTER_OCCUPANCY k_raster_meshlets(lh_s
     const Segments<VS> seg(vs, ps);
     const uint32_t total = seg.total();
     Tally<ST> tally;
-    for (uint32_t g = blockIdx.x; g < total; g += gridDim.x) {
-        uint32_t view, pi, d;
-        seg.find(g, view, pi, d);
+    auto body = [&](uint32_t view, uint32_t pi, uint32_t d) {
         const lh_view& v = vs.get(view);
         const Pass p = make_pass(v, ps.pass0 + pi);
         const Queues qs = queues(v);
@@ -458,6 +475,24 @@ __global__ void __launch_bounds__(64) LH_RASTER_OCCUPANCY k_raster_meshlets(lh_s
         }
         tally.flush(v, p.pass);
         __syncthreads();
+    };
+    constexpr uint32_t C = Segments<VS>::kChunk;
+    if constexpr (C == 1) {
+        for (uint32_t g = blockIdx.x; g < total; g += gridDim.x) {
+            uint32_t view, pi, d;
+            seg.find(g, view, pi, d);
+            body(view, pi, d);
+        }
+    } else {
+        for (uint32_t c0 = blockIdx.x * C; c0 < total; c0 += gridDim.x * C) {
+            auto cur = seg.cursor(c0);
+            const uint32_t c1 = min(c0 + C, total);
+            for (uint32_t g = c0; g < c1; g++) {
+                uint32_t view, pi, d;
+                seg.at(cur, g, view, pi, d);
+                body(view, pi, d);
+            }
+        }
     }
 }
 
