@@ -19,7 +19,7 @@ ROOT = HERE.parent
 
 def build() -> Path:
     out = HERE / "liblh_oracle.so"
-    cmd = ["g++", "-O2", "-std=c++20", "-fPIC", "-shared", "-ffp-contract=off", "-fno-fast-math", "-Wall",
+    cmd = ["g++", "-O2", "-std=c++20", "-fPIC", "-shared", "-pthread", "-ffp-contract=off", "-fno-fast-math", "-Wall",
            f"-I{ROOT / 'include'}", f"-I{HERE}", str(HERE / "oracle.cpp"), str(HERE / "oracle64.cpp"), "-o",
            str(out)]
     cmd = [c for c in cmd if not c.endswith("oracle64.cpp") or Path(c).exists()]

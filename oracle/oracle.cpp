@@ -12,8 +12,10 @@
 #include "oracle.h"
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstring>
+#include <thread>
 #include <vector>
 
 namespace {
@@ -317,7 +319,20 @@ struct RasterTarget {
     uint32_t W, H, kind;  // kind 0 main (u64 keys), 1 shadow (u32 depth)
     uint64_t* vis;
     uint32_t* depth;
+    bool shared = false;  // several threads rasterize into the target: min by compare-exchange
 };
+
+template <class T>
+inline void store_min(T& slot, T value, bool shared) {
+    if (!shared) {
+        if (value < slot) slot = value;
+        return;
+    }
+    std::atomic_ref<T> a(slot);
+    T cur = a.load(std::memory_order_relaxed);
+    while (value < cur && !a.compare_exchange_weak(cur, value, std::memory_order_relaxed)) {
+    }
+}
 
 inline int64_t edge(int32_t ax, int32_t ay, int32_t bx, int32_t by, int32_t px, int32_t py) {
     return int64_t(bx - ax) * int64_t(py - ay) - int64_t(by - ay) * int64_t(px - ax);
@@ -366,10 +381,10 @@ uint64_t raster_snapped(SV v0, SV v1, SV v2, const RasterTarget& t, uint32_t id)
             if (t.kind == 0) {
                 if (!(z > 0.0f)) continue;  // depth GREATER against the 0.0 clear
                 uint64_t key = (uint64_t(~fbits(z)) << 32) | id;
-                if (key < t.vis[p]) t.vis[p] = key;
+                store_min(t.vis[p], key, t.shared);
             } else {
                 uint32_t d = fbits(z);
-                if (d < t.depth[p]) t.depth[p] = d;
+                store_min(t.depth[p], d, t.shared);
             }
         }
     }
@@ -397,11 +412,9 @@ uint64_t raster_clip_triangle(const CV* c, const RasterTarget& t, uint32_t id) {
     return frags;
 }
 
-// Rasterizes every triangle of every meshlet in draws[pass]
-// (drawIndirectCount of src/rendering.cpp:235-242 / :405-412).
-void raster_pass(const lh_scene& sc, const lh_view& v, uint32_t pass) {
+// Draws [d0, d1) of pass `pass`; returns the covered samples.
+uint64_t raster_draws(const lh_scene& sc, const lh_view& v, uint32_t pass, uint32_t d0, uint32_t d1, bool shared) {
     const Uniforms& u = *v.uniforms;
-    uint32_t count = std::min(v.counters[LH_CNT_DRAWS + pass], v.draw_capacity);
     const MeshletReference* draws = v.draws + size_t(pass) * v.draw_capacity;
     RasterTarget t;
     const LhMat4* m;
@@ -414,8 +427,9 @@ void raster_pass(const lh_scene& sc, const lh_view& v, uint32_t pass) {
                          v.shadow_maps + size_t(c) * v.shadow_size * v.shadow_size};
         m = &v.misc->shadow_matrices[c];
     }
+    t.shared = shared;
     uint64_t frags = 0;
-    for (uint32_t d = 0; d < count; d++) {
+    for (uint32_t d = d0; d < d1; d++) {
         const MeshletReference& ref = draws[d];
         const Instance& in = sc.instances[ref.instance_index];
         const MeshInfo& mi = mesh_info_of(in);
@@ -432,9 +446,40 @@ void raster_pass(const lh_scene& sc, const lh_view& v, uint32_t pass) {
             frags += raster_clip_triangle(c, t, (mid << LH_KEY_TRI_BITS) | k);
         }
     }
+    return frags;
+}
+
+// Splits [0, n) into `threads` contiguous parts run on std::threads.
+template <class F>
+void parallel_for(uint32_t n, uint32_t threads, F&& f) {
+    threads = std::max(1u, std::min(threads, std::max(n, 1u)));
+    if (threads == 1) {
+        f(0u, n, 0u);
+        return;
+    }
+    std::vector<std::thread> pool;
+    for (uint32_t i = 0; i < threads; i++) {
+        const uint32_t a = uint32_t(uint64_t(n) * i / threads), b = uint32_t(uint64_t(n) * (i + 1) / threads);
+        pool.emplace_back([&f, a, b, i] { f(a, b, i); });
+    }
+    for (auto& th : pool) th.join();
+}
+
+// Rasterizes every triangle of every meshlet in draws[pass]
+// (drawIndirectCount of src/rendering.cpp:235-242 / :405-412), with
+// `threads` threads over contiguous draw ranges (the result is a minimum per
+// pixel, so it does not depend on the split).
+void raster_pass(const lh_scene& sc, const lh_view& v, uint32_t pass, uint32_t threads = 1) {
+    const uint32_t count = std::min(v.counters[LH_CNT_DRAWS + pass], v.draw_capacity);
+    std::vector<uint64_t> frags(std::max(threads, 1u), 0);
+    parallel_for(count, threads, [&](uint32_t a, uint32_t b, uint32_t i) {
+        frags[i] = raster_draws(sc, v, pass, a, b, threads > 1);
+    });
+    uint64_t total = 0;
+    for (uint64_t f : frags) total += f;
     // LH_VIEW_STATISTICS: covered samples of the pass (depth-test order
     // dependent atomics counts have no sequential counterpart).
-    if (v.flags & LH_VIEW_STATISTICS) v.counters[LH_CNT_FRAGMENTS + pass] += uint32_t(frags);
+    if (v.flags & LH_VIEW_STATISTICS) v.counters[LH_CNT_FRAGMENTS + pass] += uint32_t(total);
 }
 
 // ------------------------------------------------------------- resolve
@@ -1044,9 +1089,8 @@ lh_status orc_cull_instances_shadow(const lh_scene* sc, const lh_view* v) {
     return LH_OK;
 }
 
-lh_status orc_resolve(const lh_scene* sc, const lh_view* v) {
-    if (!valid(sc, v) || !v->visbuffer || !v->radiance) return LH_ERR_INVALID;
-    for (uint32_t y = 0; y < v->height; y++)
+static void resolve_rows(const lh_scene* sc, const lh_view* v, uint32_t y0, uint32_t y1) {
+    for (uint32_t y = y0; y < y1; y++)
         for (uint32_t x = 0; x < v->width; x++) {
             F3 c = resolve_pixel(*sc, *v, x, y);
             float* o = v->radiance + 4 * (size_t(y) * v->width + x);
@@ -1056,6 +1100,11 @@ lh_status orc_resolve(const lh_scene* sc, const lh_view* v) {
             o[3] = 1.0f;
             if (v->ldr) v->ldr[size_t(y) * v->width + x] = tonemap_pixel(c);
         }
+}
+
+lh_status orc_resolve(const lh_scene* sc, const lh_view* v) {
+    if (!valid(sc, v) || !v->visbuffer || !v->radiance) return LH_ERR_INVALID;
+    resolve_rows(sc, v, 0, v->height);
     return LH_OK;
 }
 
@@ -1083,6 +1132,27 @@ lh_status orc_frame(const lh_scene* sc, const lh_view* v) {
         if ((s = orc_raster_shadow(sc, v, c))) return s;
     }
     if ((s = orc_resolve(sc, v))) return s;
+    return v->counters[LH_CNT_OVERFLOW] ? LH_ERR_CAPACITY : LH_OK;
+}
+
+// orc_frame with the rasters (contiguous draw ranges, minimum by
+// compare-exchange) and the resolve (row bands) on `threads` std::threads;
+// identical outputs.  The CPU baseline of the benchmark.
+lh_status orc_frame_threads(const lh_scene* sc, const lh_view* v, uint32_t threads) {
+    lh_status s;
+    if (!valid(sc, v) || !v->visbuffer || !v->shadow_maps || !v->radiance) return LH_ERR_INVALID;
+    if ((s = orc_view_reset(sc, v))) return s;
+    if ((s = orc_cull_instances(sc, v))) return s;
+    if ((s = orc_cull_meshlets(sc, v, 0))) return s;
+    raster_pass(*sc, *v, 0, threads);
+    if ((s = orc_depth_bounds(sc, v))) return s;
+    if ((s = orc_fit_cascades(sc, v))) return s;
+    if ((s = orc_cull_instances_shadow(sc, v))) return s;
+    for (uint32_t c = 0; c < LH_NUM_CASCADES; c++) {
+        if ((s = orc_cull_meshlets(sc, v, 1 + c))) return s;
+        raster_pass(*sc, *v, 1 + c, threads);
+    }
+    parallel_for(v->height, threads, [&](uint32_t a, uint32_t b, uint32_t) { resolve_rows(sc, v, a, b); });
     return v->counters[LH_CNT_OVERFLOW] ? LH_ERR_CAPACITY : LH_OK;
 }
 

@@ -38,6 +38,8 @@ lh_status orc_raster_shadow(const lh_scene* scene, const lh_view* view, uint32_t
 lh_status orc_resolve(const lh_scene* scene, const lh_view* view);
 lh_status orc_tonemap(const lh_view* view);
 lh_status orc_frame(const lh_scene* scene, const lh_view* view);
+/* orc_frame with the rasters and the resolve on `threads` threads (same outputs). */
+lh_status orc_frame_threads(const lh_scene* scene, const lh_view* view, uint32_t threads);
 
 /* ---- building blocks exposed for known-answer tests ---- */
 

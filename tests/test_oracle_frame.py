@@ -133,3 +133,26 @@ def oracle_material():
     from lighthugger_amd.scene import Material
 
     return Material(base_color=(0.5, 0.5, 0.5), roughness=0.5, metallic=0.0)
+
+
+def test_threaded_oracle_frame_is_identical():
+    """orc_frame_threads (the CPU baseline) produces the scalar frame bit for bit."""
+    import numpy as np
+
+    from lighthugger_amd import abi
+    from lighthugger_amd.frame import ViewConfig
+    from lighthugger_amd.scene import HostScene, make_camera, make_uniforms, preset_camera
+    from oracle import lib
+    from oracle.compare import oracle_view, snapshot
+
+    scene = HostScene.preset(abi.PRESET_LARGE, num_instances=300, texture_size=32)
+    base = preset_camera(abi.PRESET_LARGE)
+    cam = make_camera(base.position.tolist(), base.target.tolist(), 320, 200)
+    cfg = ViewConfig(320, 200, 256)
+    sd, vd = oracle_view(scene, cfg, make_uniforms(cam, scene.num_instances))
+    lib().orc_frame(sd.address, vd.address)
+    a = snapshot(vd)
+    lib().orc_frame_threads(sd.address, vd.address, 4)
+    b = snapshot(vd)
+    for k in ("visbuffer", "shadow", "radiance", "ldr", "misc", "counters"):
+        assert np.array_equal(a[k], b[k]), k
