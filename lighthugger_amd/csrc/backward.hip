@@ -474,15 +474,27 @@ __global__ void __launch_bounds__(64) LH_BACKWARD_OCCUPANCY k_resolve_backward(l
     }
 }
 
-// Sums the per-block light partials in block order (deterministic) and adds
-// them to the outputs.
-__global__ void __launch_bounds__(64) k_reduce_light(const double* partials, uint32_t blocks, lh_resolve_grads gr) {
-    const uint32_t i = threadIdx.x;
-    if (i >= 6) return;
-    double sum = 0.0;
-    for (uint32_t b = 0; b < blocks; b++) sum += partials[size_t(b) * 8 + i];
-    float* dst = i < 3 ? gr.sun_dir : gr.sun_intensity;
-    if (dst) dst[i % 3] += float(sum);
+// Sums the per-block light partials with a fixed reduction tree (the result
+// does not depend on timing) and adds them to the outputs: 256 threads each
+// sum a strided subset per component, then a tree in LDS.
+__global__ void __launch_bounds__(256) k_reduce_light(const double* partials, uint32_t blocks, lh_resolve_grads gr) {
+    __shared__ double acc[6][256];
+    const uint32_t t = threadIdx.x;
+    for (int i = 0; i < 6; i++) {
+        double sum = 0.0;
+        for (uint32_t b = t; b < blocks; b += 256) sum += partials[size_t(b) * 8 + i];
+        acc[i][t] = sum;
+    }
+    __syncthreads();
+    for (uint32_t w = 128; w >= 1; w >>= 1) {
+        if (t < w)
+            for (int i = 0; i < 6; i++) acc[i][t] += acc[i][t + w];
+        __syncthreads();
+    }
+    if (t < 6) {
+        float* dst = t < 3 ? gr.sun_dir : gr.sun_intensity;
+        if (dst) dst[t % 3] += float(acc[t][0]);
+    }
 }
 
 }  // namespace
@@ -501,7 +513,7 @@ hipError_t resolve_backward(const lh_scene& s, const OneView& vs, const lh_resol
     hipLaunchKernelGGL(k_resolve_backward, dim3(blocks), dim3(64), 0, st, s, vs, params, grad_radiance, grads,
                        partials);
     mark("reduce_light", st);
-    hipLaunchKernelGGL(k_reduce_light, dim3(1), dim3(64), 0, st, partials, blocks, grads);
+    hipLaunchKernelGGL(k_reduce_light, dim3(1), dim3(256), 0, st, partials, blocks, grads);
     return hipGetLastError();
 }
 
