@@ -404,6 +404,15 @@ __global__ void __launch_bounds__(64) LH_RASTER_OCCUPANCY k_raster_meshlets(lh_s
         const Meshlet& ml = ptr<Meshlet>(mi.meshlets)[ref.meshlet_index];
         const uint32_t mid = s.instance_meshlet_base[ref.instance_index] + ref.meshlet_index;
         const uint32_t nvert = ml.index_count, ntri = ml.triangle_count;
+        // both rounds' corner indices, issued before the vertex fetch so the
+        // two latencies overlap
+        const uint8_t* micro = ptr<uint8_t>(mi.micro_indices) + ml.triangle_offset;
+        uint32_t corners[2] = {0u, 0u};
+        for (uint32_t r = 0; r < 2; r++) {
+            const uint32_t t = lane + 64 * r;
+            if (t < ntri) corners[r] = uint32_t(micro[3 * t]) | (uint32_t(micro[3 * t + 1]) << 8) |
+                                       (uint32_t(micro[3 * t + 2]) << 16);
+        }
         if (lane < nvert) {
             f3 w = mat4_point(in.transform, load_position(mi, load_index(mi, ml.index_offset + lane)));
             f4 c = mat4_mul_vec4(M, f4{w.x, w.y, w.z, 1.0f});
@@ -421,7 +430,6 @@ __global__ void __launch_bounds__(64) LH_RASTER_OCCUPANCY k_raster_meshlets(lh_s
             sO[lane] = oc;
         }
         __syncthreads();
-        const uint8_t* micro = ptr<uint8_t>(mi.micro_indices) + ml.triangle_offset;
         for (uint32_t round = 0; round < 2; round++) {
             const uint32_t t = lane + 64 * round;
             if (__ballot(t < ntri) == 0) break;
@@ -431,7 +439,8 @@ __global__ void __launch_bounds__(64) LH_RASTER_OCCUPANCY k_raster_meshlets(lh_s
             int64_t area = 0;
             int32_t xlo = 0, xhi = 0, ylo = 0, yhi = 0, ex = 0, ey = 0;
             if (t < ntri) {
-                uint32_t i0 = micro[3 * t], i1 = micro[3 * t + 1], i2 = micro[3 * t + 2];
+                const uint32_t packed = corners[round];
+                uint32_t i0 = packed & 255u, i1 = (packed >> 8) & 255u, i2 = packed >> 16;
                 uint32_t o0 = sO[i0], o1 = sO[i1], o2 = sO[i2];
                 if (((o0 | o1 | o2) & kBadVertex) == 0 && (o0 & o1 & o2) == 0) {
                     if (o0 | o1 | o2) {
