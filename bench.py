@@ -241,7 +241,7 @@ def diff_bench(steps: int, tri_scale_pct: int = 18, side: int = 1024) -> dict:
     def backward():
         for t in out.values():
             t.zero_()
-        d.raw_backward(p, g, out)
+        d.raw_backward(p, g, out, recorded=True)  # fixed quantities recorded by the forward
 
     fwd = timed(lambda: d.forward_raw(p))
     bwd = timed(backward)

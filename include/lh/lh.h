@@ -176,6 +176,10 @@ typedef struct lh_resolve_params {
     const float* metallic_factor;   /* [num_mesh_infos] */
     const float* sun_dir;           /* [3]                   (Uniforms.sun_dir, used as given) */
     const float* sun_intensity;     /* [3]                   (Uniforms.sun_intensity) */
+    uint32_t* frozen;               /* optional [width * height * 4]: lh_resolve_ex stores, per pixel, the
+                                       quantities the adjoint holds fixed (shadow factor, back-face flip,
+                                       texture LOD); lh_resolve_backward then reads them instead of
+                                       recomputing them.  Pass the buffer the forward filled. */
 } lh_resolve_params;
 lh_status lh_resolve_ex(const lh_scene* scene, const lh_view* view, const lh_resolve_params* params,
                         lh_stream stream);

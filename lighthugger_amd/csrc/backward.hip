@@ -288,9 +288,22 @@ LH_DEV void pixel_bwd(const lh_scene& s, const lh_view& v, const lh_resolve_para
         atomicOr(&v.counters[LH_CNT_FLAGS], LH_FLAG_BACKWARD_UNSUPPORTED);
         return;
     }
-    const f2 win{float(u.window_size.x), float(u.window_size.y)};
-    const f2 ndc{float(x) / win.x * 2.0f - 1.0f, float(y) / win.y * 2.0f - 1.0f};
-    const Frozen fz = frozen(s, v, mi, sf, ndc, win);
+    Frozen fz;
+    if (prm.frozen) {  // recorded by the forward (lh_resolve_ex with the same buffer)
+        const uint4 r = reinterpret_cast<const uint4*>(prm.frozen)[o];
+        fz.flip = (r.w & 1u) ? -1.0f : 1.0f;
+        fz.K = kPi * __uint_as_float(r.x);
+        auto lod = [&](uint32_t index, uint32_t l0, uint32_t fbits) {
+            if (!has_texture(s, index)) return Lod{0, 0, 0.0f};
+            return Lod{l0, min(l0 + 1, s.textures[index].levels - 1), __uint_as_float(fbits)};
+        };
+        fz.lod_b = lod(mi.base_color_texture_index, (r.w >> 8) & 255u, r.y);
+        fz.lod_m = lod(mi.metallic_roughness_texture_index, (r.w >> 16) & 255u, r.z);
+    } else {
+        const f2 win{float(u.window_size.x), float(u.window_size.y)};
+        const f2 ndc{float(x) / win.x * 2.0f - 1.0f, float(y) / win.y * 2.0f - 1.0f};
+        fz = frozen(s, v, mi, sf, ndc, win);
+    }
     const float4 gin = reinterpret_cast<const float4*>(grad_radiance)[o];
     const d3 g{gin.x, gin.y, gin.z};
     // ---- forward in FP64 ----

@@ -149,13 +149,27 @@ LH_DEV bool has_texture(const lh_scene& s, uint32_t index) {
 // sample_texture (render_geometry.comp:59-66) as isotropic trilinear
 // filtering with the LOD from the analytic uv derivatives.  An unused index
 // reads as opaque white.
-LH_DEV f4 sample_texture(const lh_scene& s, uint32_t index, const Interp2& uv) {
+LH_DEV f4 sample_texture_lod(const lh_scene& s, uint32_t index, const Lod& lod, const Interp2& uv) {
     if (!has_texture(s, index)) return {1, 1, 1, 1};
     const LhTexture& t = s.textures[index];
-    const Lod lod = texture_lod(t, uv);
     const f4 c0 = bilinear(t, lod.l0, uv.v);
     if (lod.f == 0.0f) return c0;
     return mix4(c0, bilinear(t, lod.l1, uv.v), lod.f);
+}
+LH_DEV Lod lod_of(const lh_scene& s, uint32_t index, const Interp2& uv) {
+    return has_texture(s, index) ? texture_lod(s.textures[index], uv) : Lod{0, 0, 0.0f};
+}
+LH_DEV f4 sample_texture(const lh_scene& s, uint32_t index, const Interp2& uv) {
+    return sample_texture_lod(s, index, lod_of(s, index, uv), uv);
+}
+
+// Per-pixel record of the quantities the adjoint holds fixed
+// (lh_resolve_params.frozen): shadow factor, LOD blends of the base-colour
+// and metallic-roughness textures, and a word with the flip (bit 0), their
+// base levels (bits 8-15, 16-23) and a valid bit (31).
+LH_DEV uint4 pack_frozen(float shadow_sum, bool flip, const Lod& b, const Lod& m) {
+    return make_uint4(__float_as_uint(shadow_sum), __float_as_uint(b.f), __float_as_uint(m.f),
+                      (flip ? 1u : 0u) | (b.l0 << 8) | (m.l0 << 16) | 0x80000000u);
 }
 
 // sampler2DArrayShadow tap: linear filter, clamp to edge, compare LESS.
@@ -328,11 +342,15 @@ LH_DEV f3 resolve_pixel(const lh_scene& s, const lh_view& v, uint32_t x, uint32_
                  .v;
     nrm = normalize3(mat3_mul_vec3(in.normal_transform, nrm));
     const f3 gn = cross3(sub(sf.pos[1], sf.pos[0]), sub(sf.pos[2], sf.pos[0]));
-    if (dot3(gn, view_vector) < 0.0f) nrm = neg(nrm);
+    const bool flip = dot3(gn, view_vector) < 0.0f;
+    if (flip) nrm = neg(nrm);
     const float shadow_sum = shadow_factor(v, mi, world.v);
-    const f4 bc = sample_texture(s, mi.base_color_texture_index, uv);
+    const Lod lod_b = lod_of(s, mi.base_color_texture_index, uv), lod_m = lod_of(s, mi.metallic_roughness_texture_index, uv);
+    if (prm.frozen)
+        reinterpret_cast<uint4*>(prm.frozen)[size_t(y) * v.width + x] = pack_frozen(shadow_sum, flip, lod_b, lod_m);
+    const f4 bc = sample_texture_lod(s, mi.base_color_texture_index, lod_b, uv);
     const f3 base = mul(f3{bc.x, bc.y, bc.z}, param_base_color(prm, mi, sf.mesh_info));
-    const f4 mr = sample_texture(s, mi.metallic_roughness_texture_index, uv);
+    const f4 mr = sample_texture_lod(s, mi.metallic_roughness_texture_index, lod_m, uv);
     const float roughness = mr.y * param_roughness(prm, s, sf.mesh_info);
     const float metallic = mr.z * param_metallic(prm, s, sf.mesh_info);
     if (mi.normal_texture_index != UNUSED_TEXTURE_INDEX) {

@@ -33,9 +33,10 @@ void check_param(const Tensor& t, int64_t n, const char* what) {
     TORCH_CHECK(t.numel() == n, what, " has ", t.numel(), " elements, expected ", n);
 }
 
-lh_resolve_params params_of(const Tensor& bc, const Tensor& r, const Tensor& m, const Tensor& sd, const Tensor& si) {
+lh_resolve_params params_of(const Tensor& bc, const Tensor& r, const Tensor& m, const Tensor& sd, const Tensor& si,
+                            const Tensor& frozen) {
     return lh_resolve_params{bc.data_ptr<float>(), r.data_ptr<float>(), m.data_ptr<float>(), sd.data_ptr<float>(),
-                             si.data_ptr<float>()};
+                             si.data_ptr<float>(), reinterpret_cast<uint32_t*>(frozen.data_ptr<int32_t>())};
 }
 
 class ResolveFunction : public torch::autograd::Function<ResolveFunction> {
@@ -56,7 +57,10 @@ class ResolveFunction : public torch::autograd::Function<ResolveFunction> {
                     "geometry must be the scene's geometry arena as contiguous float32 words");
         TORCH_CHECK(uint64_t(geometry.data_ptr<float>()) == uint64_t(geometry_base),
                     "geometry must alias the scene's geometry arena");
-        const lh_resolve_params p = params_of(base_color, roughness, metallic, sun_dir, sun_intensity);
+        // per-pixel quantities the adjoint holds fixed, recorded by the forward
+        Tensor frozen = torch::empty({int64_t(v->height) * v->width * 4},
+                                     torch::TensorOptions().dtype(torch::kInt32).device(base_color.device()));
+        const lh_resolve_params p = params_of(base_color, roughness, metallic, sun_dir, sun_intensity, frozen);
         const lh_status st = lh_resolve_ex(s, v, &p, stream_of(stream));
         TORCH_CHECK(st == LH_OK, "lh_resolve_ex failed with status ", st);
         auto opts = torch::TensorOptions().dtype(torch::kFloat32).device(base_color.device());
@@ -65,7 +69,7 @@ class ResolveFunction : public torch::autograd::Function<ResolveFunction> {
         ctx->saved_data["view"] = view;
         ctx->saved_data["stream"] = stream;
         ctx->saved_data["geometry_base"] = geometry_base;
-        ctx->save_for_backward({geometry, base_color, roughness, metallic, sun_dir, sun_intensity});
+        ctx->save_for_backward({geometry, base_color, roughness, metallic, sun_dir, sun_intensity, frozen});
         return out;
     }
 
@@ -76,12 +80,12 @@ class ResolveFunction : public torch::autograd::Function<ResolveFunction> {
         const uint64_t geometry_base = uint64_t(ctx->saved_data["geometry_base"].toInt());
         auto saved = ctx->get_saved_variables();
         const Tensor &geometry = saved[0], &bc = saved[1], &r = saved[2], &m = saved[3], &sd = saved[4],
-                     &si = saved[5];
+                     &si = saved[5], &frozen = saved[6];
         Tensor g = grad_outputs[0].contiguous();
         TORCH_CHECK(g.numel() == int64_t(v->width) * v->height * 4, "radiance gradient of the wrong size");
         Tensor g_geo = torch::zeros_like(geometry), g_bc = torch::zeros_like(bc), g_r = torch::zeros_like(r),
                g_m = torch::zeros_like(m), g_sd = torch::zeros_like(sd), g_si = torch::zeros_like(si);
-        const lh_resolve_params p = params_of(bc, r, m, sd, si);
+        const lh_resolve_params p = params_of(bc, r, m, sd, si, frozen);
         const lh_resolve_grads gr{g_bc.data_ptr<float>(), g_r.data_ptr<float>(), g_m.data_ptr<float>(),
                                   g_sd.data_ptr<float>(), g_si.data_ptr<float>(), g_geo.data_ptr<float>(),
                                   geometry_base, uint64_t(geometry.numel()) * 4};

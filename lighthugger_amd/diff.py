@@ -25,7 +25,7 @@ class ResolveParams(C.Structure):
     """lh_resolve_params (include/lh/lh.h)."""
 
     _fields_ = [("base_color_factor", C.c_void_p), ("roughness_factor", C.c_void_p), ("metallic_factor", C.c_void_p),
-                ("sun_dir", C.c_void_p), ("sun_intensity", C.c_void_p)]
+                ("sun_dir", C.c_void_p), ("sun_intensity", C.c_void_p), ("frozen", C.c_void_p)]
 
 
 class ResolveGrads(C.Structure):
@@ -75,6 +75,10 @@ class DifferentiableResolve:
         self.torch = torch
         self.r = renderer
         self.geometry = renderer.scene.geometry_words()  # aliases the scene's geometry arena
+        cfg = renderer.view.cfg
+        # per-pixel quantities the adjoint holds fixed, recorded by forward_raw
+        self.frozen = torch.empty(cfg.width * cfg.height * 4, dtype=torch.int32, device=self.geometry.device)
+        self._recorded = False
 
     def parameters(self, requires_grad: bool = False) -> dict:
         """The differentiable parameters as the scene and the view's uniforms hold them."""
@@ -99,19 +103,24 @@ class DifferentiableResolve:
                                    self.r.scene.geometry_base, base_color.contiguous(), roughness.contiguous(),
                                    metallic.contiguous(), sun_dir.contiguous(), sun_intensity.contiguous())
 
-    @staticmethod
-    def _params_struct(p: dict) -> ResolveParams:
-        return ResolveParams(*[int(p[k].data_ptr()) for k in PARAM_NAMES])
+    def _params_struct(self, p: dict, frozen: bool) -> ResolveParams:
+        return ResolveParams(*[int(p[k].data_ptr()) for k in PARAM_NAMES],
+                             int(self.frozen.data_ptr()) if frozen else 0)
 
     def forward_raw(self, p: dict) -> None:
-        """lh_resolve_ex into the view's radiance buffer."""
-        s = self._params_struct(p)
+        """lh_resolve_ex into the view's radiance buffer; records the adjoint's
+        fixed quantities for raw_backward(..., recorded=True)."""
+        s = self._params_struct(p, True)
         check(_bind().lh_resolve_ex(self.r.scene.address, self.r.view.address, C.byref(s), current_stream()),
               "lh_resolve_ex")
+        self._recorded = True
 
-    def raw_backward(self, p: dict, grad_radiance, out: dict | None = None) -> dict:
+    def raw_backward(self, p: dict, grad_radiance, out: dict | None = None, recorded: bool = False) -> dict:
         """Gradients of sum(grad_radiance * radiance) through lh_resolve_backward
-        (added into `out` when given, else into fresh zero tensors)."""
+        (added into `out` when given, else into fresh zero tensors).  With
+        `recorded`, the fixed quantities come from the last forward_raw."""
+        if recorded and not self._recorded:
+            raise RuntimeError("raw_backward(recorded=True) needs a forward_raw first")
         t = self.torch
         if out is None:
             out = {k: t.zeros_like(p[k]) for k in PARAM_NAMES}
@@ -120,7 +129,7 @@ class DifferentiableResolve:
         assert g.dtype == t.float32 and g.numel() == self.r.view.cfg.width * self.r.view.cfg.height * 4
         grads = ResolveGrads(*[int(out[k].data_ptr()) for k in PARAM_NAMES], int(out["geometry"].data_ptr()),
                              self.r.scene.geometry_base, int(out["geometry"].numel()) * 4)
-        s = self._params_struct(p)
+        s = self._params_struct(p, recorded)
         check(_bind().lh_resolve_backward(self.r.scene.address, self.r.view.address, C.byref(s), int(g.data_ptr()),
                                           C.byref(grads), current_stream()), "lh_resolve_backward")
         return out

@@ -103,6 +103,13 @@ def test_backward_matches_fp64_adjoint():
     assert np.linalg.norm(pos64) > 0 and np.linalg.norm(nrm64) > 0
     for k, e in errs.items():
         assert e <= REL_L2, (k, e)
+    # the same with the fixed quantities recorded by the forward
+    d.forward_raw(p)
+    rec = d.raw_backward(p, torch.from_numpy(g).cuda(), recorded=True)
+    torch.cuda.synchronize()
+    got_r = {k: v.cpu().numpy().astype(np.float64) for k, v in rec.items()}
+    for k in ("base_color", "roughness", "metallic", "sun_dir", "sun_intensity", "geometry"):
+        assert _rel(got_r[k], got[k]) <= 1e-6, k
 
 
 def test_autograd_function_l2_loss():
