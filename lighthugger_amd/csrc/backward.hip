@@ -6,7 +6,13 @@
 // mesh-space vertex positions and normals of the triangle each pixel shows.
 // Held fixed, like the visibility buffer itself: the back-face flip, the
 // cascade choice and PCF shadow factor, and the texture LOD (levels and blend
-// weight), all recomputed here exactly as the forward computes them.
+// weight), read from the forward's per-pixel record when the caller kept one
+// (lh_resolve_params.frozen), else recomputed here exactly as the forward
+// computes them.
+//
+// Precision: the clip-space chain (positions, clip coordinates, barycentrics
+// and their reverse) and the texel-cell choice run in FP64 from the FP32
+// inputs; the BRDF, texture filtering, normals and light run in FP32.
 //
 // Barycentrics are not stored: they are recomputed from the key.  Their
 // adjoint uses the homogeneous form of the perspective-correct barycentrics
@@ -46,146 +52,146 @@ LH_DEV double dot(d3 a, d3 b) { return (a.x * b.x + a.y * b.y) + a.z * b.z; }
 LH_DEV d3 cross(d3 a, d3 b) { return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x}; }
 LH_DEV double len(d3 a) { return sqrt(dot(a, a)); }
 LH_DEV d3 normalize(d3 a) { return a * (1.0 / len(a)); }
-LH_DEV double clampd(double x, double lo, double hi) { return fmin(fmax(x, lo), hi); }
-LH_DEV double pow4(double x) {
-    const double x2 = x * x;
-    return x2 * x2;
-}
-LH_DEV double pow5(double x) { return pow4(x) * x; }
 // columns 0..2 of a 4x4 (column-major) applied to a point / the linear part
 LH_DEV d3 col3(const LhVec4& c) { return {double(c.x), double(c.y), double(c.z)}; }
 
+// ---- FP32 part: BRDF, textures, normals ----
+LH_DEV float pow4f(float x) {
+    const float x2 = x * x;
+    return x2 * x2;
+}
+LH_DEV float pow5f(float x) { return pow4f(x) * x; }
+LH_DEV f3 madd(f3 a, f3 b, float s) { return add(a, mul(b, s)); }
+
 // u = a / |a|: the gradient of a from the gradient of u.
-LH_DEV d3 normalize_bwd(d3 a, d3 gu) {
-    const double l = len(a);
-    const d3 u = a * (1.0 / l);
-    return (gu - u * dot(u, gu)) * (1.0 / l);
+LH_DEV f3 normalize_bwd(f3 a, f3 gu) {
+    const float l = length3(a);
+    const f3 u = mul(a, 1.0f / l);
+    return mul(sub(gu, mul(u, dot3(u, gu))), 1.0f / l);
 }
 
 struct BrdfGrad {
-    d3 v, l, n, base;
-    double pr, metallic;
+    f3 v, l, n, base;
+    float pr, metallic;
 };
 
-// BRDF of shading::brdf (thin = false) in FP64: value and reverse mode for the
+// BRDF of shading::brdf (thin = false): value and reverse mode for the
 // upstream gradient g of its RGB output.
-LH_DEV d3 brdf_bwd(d3 v, d3 l, d3 n, double pr, double m, d3 base, d3 g, BrdfGrad& o) {
-    const d3 hv = v + l;
-    const d3 h = normalize(hv);
-    const d3 diffuse = base * (1.0 - m);
-    const double refl = (0.5 / 2.5) * (0.5 / 2.5);
-    const double d0 = refl * (1.0 - m);
-    const d3 f0 = d3{d0, d0, d0} + base * m;
-    const double nv = dot(n, v), nl = dot(n, l), nh = dot(n, h), lh = dot(l, h);
-    const double NoV = clampd(nv, 1e-5, 1.0), NoL = clampd(nl, 0.0, 1.0);
-    const double NoH = clampd(nh, 0.0, 1.0), LoH = clampd(lh, 0.0, 1.0);
-    const double a = pr * pr, a2 = a * a;
-    const double fD = (NoH * a2 - NoH) * NoH + 1.0;
-    const double D = a2 / ((M_PI * fD) * fD);
-    const double p = pow5(1.0 - LoH);
-    const d3 F{f0.x + (1.0 - f0.x) * p, f0.y + (1.0 - f0.y) * p, f0.z + (1.0 - f0.z) * p};
-    const double sL = sqrt((-NoL * a2 + NoL) * NoL + a2), sV = sqrt((-NoV * a2 + NoV) * NoV + a2);
-    const double den = NoL * sV + NoV * sL;
-    const double Vis = 0.5 / den;
-    const double f90 = 0.5 + ((2.0 * a) * LoH) * LoH;
-    const double q5L = pow5(1.0 - NoL), q5V = pow5(1.0 - NoV);
-    const double FSL = 1.0 + (f90 - 1.0) * q5L, FSV = 1.0 + (f90 - 1.0) * q5V;
-    const double FdB = (FSL * FSV) * (1.0 / M_PI);
-    const double DV = D * Vis;
-    const d3 Fr = F * DV, Fd = diffuse * FdB;
-    const d3 irr = (Fd + Fr) * NoL;
+LH_DEV f3 brdf_bwd(f3 v, f3 l, f3 n, float pr, float m, f3 base, f3 g, BrdfGrad& o) {
+    const f3 hv = add(v, l);
+    const f3 h = normalize3(hv);
+    const f3 diffuse = mul(base, 1.0f - m);
+    const float refl = (0.5f / 2.5f) * (0.5f / 2.5f);
+    const float d0 = refl * (1.0f - m);
+    const f3 f0 = madd(f3{d0, d0, d0}, base, m);
+    const float nv = dot3(n, v), nl = dot3(n, l), nh = dot3(n, h), lh = dot3(l, h);
+    const float NoV = clampf(nv, 1e-5f, 1.0f), NoL = clampf(nl, 0.0f, 1.0f);
+    const float NoH = clampf(nh, 0.0f, 1.0f), LoH = clampf(lh, 0.0f, 1.0f);
+    const float a = pr * pr, a2 = a * a;
+    const float fD = (NoH * a2 - NoH) * NoH + 1.0f;
+    const float D = a2 / ((kPi * fD) * fD);
+    const float p = pow5f(1.0f - LoH);
+    const f3 F{f0.x + (1.0f - f0.x) * p, f0.y + (1.0f - f0.y) * p, f0.z + (1.0f - f0.z) * p};
+    const float sL = sqrtf((-NoL * a2 + NoL) * NoL + a2), sV = sqrtf((-NoV * a2 + NoV) * NoV + a2);
+    const float den = NoL * sV + NoV * sL;
+    const float Vis = 0.5f / den;
+    const float f90 = 0.5f + ((2.0f * a) * LoH) * LoH;
+    const float q5L = pow5f(1.0f - NoL), q5V = pow5f(1.0f - NoV);
+    const float FSL = 1.0f + (f90 - 1.0f) * q5L, FSV = 1.0f + (f90 - 1.0f) * q5V;
+    const float FdB = (FSL * FSV) * (1.0f / kPi);
+    const float DV = D * Vis;
+    const f3 Fr = mul(F, DV), Fd = mul(diffuse, FdB);
+    const f3 irr = mul(add(Fd, Fr), NoL);
     // max(irr, 0) and the NaN guard pass the gradient only where irr > 0
-    const d3 gi{irr.x > 0.0 ? g.x : 0.0, irr.y > 0.0 ? g.y : 0.0, irr.z > 0.0 ? g.z : 0.0};
-    const d3 gs = gi * NoL;
-    double gNoL = dot(gi, Fd + Fr);
-    const d3 gF = gs * DV;
-    const double gDV = dot(gs, F);
-    const double gD = gDV * Vis, gVis = gDV * D;
-    const d3 gdiff = gs * FdB;
-    const double gFdB = dot(gs, diffuse);
-    const double gFSL = gFdB * FSV * (1.0 / M_PI), gFSV = gFdB * FSL * (1.0 / M_PI);
-    const double gf90 = gFSL * q5L + gFSV * q5V;
-    gNoL += gFSL * (f90 - 1.0) * (-5.0 * pow4(1.0 - NoL));
-    double gNoV = gFSV * (f90 - 1.0) * (-5.0 * pow4(1.0 - NoV));
-    double ga = gf90 * (2.0 * LoH * LoH);
-    double gLoH = gf90 * (4.0 * a * LoH);
-    const double gden = -gVis * Vis / den;
+    const f3 gi{irr.x > 0.0f ? g.x : 0.0f, irr.y > 0.0f ? g.y : 0.0f, irr.z > 0.0f ? g.z : 0.0f};
+    const f3 gs = mul(gi, NoL);
+    float gNoL = dot3(gi, add(Fd, Fr));
+    const f3 gF = mul(gs, DV);
+    const float gDV = dot3(gs, F);
+    const float gD = gDV * Vis, gVis = gDV * D;
+    const f3 gdiff = mul(gs, FdB);
+    const float gFdB = dot3(gs, diffuse);
+    const float gFSL = gFdB * FSV * (1.0f / kPi), gFSV = gFdB * FSL * (1.0f / kPi);
+    const float gf90 = gFSL * q5L + gFSV * q5V;
+    gNoL += gFSL * (f90 - 1.0f) * (-5.0f * pow4f(1.0f - NoL));
+    float gNoV = gFSV * (f90 - 1.0f) * (-5.0f * pow4f(1.0f - NoV));
+    float ga = gf90 * (2.0f * LoH * LoH);
+    float gLoH = gf90 * (4.0f * a * LoH);
+    const float gden = -gVis * Vis / den;
     gNoV += gden * sL;
     gNoL += gden * sV;
-    const double gsL = gden * NoV, gsV = gden * NoL;
-    gNoL += gsL * NoL * (1.0 - a2) / sL;
-    gNoV += gsV * NoV * (1.0 - a2) / sV;
-    double ga2 = gsL * (1.0 - NoL * NoL) / (2.0 * sL) + gsV * (1.0 - NoV * NoV) / (2.0 * sV);
-    const d3 gf0 = gF * (1.0 - p);
-    const double gp = (gF.x * (1.0 - f0.x) + gF.y * (1.0 - f0.y)) + gF.z * (1.0 - f0.z);
-    gLoH += gp * (-5.0 * pow4(1.0 - LoH));
-    const double E = D / fD;
-    ga2 += gD * (1.0 / ((M_PI * fD) * fD) - 2.0 * E * NoH * NoH);
-    const double gNoH = gD * (-2.0 * E * (2.0 * NoH * (a2 - 1.0)));
-    ga += ga2 * 2.0 * a;
-    o.pr = ga * 2.0 * pr;
-    o.metallic = dot(gf0, base - d3{refl, refl, refl}) - dot(gdiff, base);
-    o.base = gf0 * m + gdiff * (1.0 - m);
-    const double gnv = (nv > 1e-5 && nv < 1.0) ? gNoV : 0.0;
-    const double gnl = (nl > 0.0 && nl < 1.0) ? gNoL : 0.0;
-    const double gnh = (nh > 0.0 && nh < 1.0) ? gNoH : 0.0;
-    const double glh = (lh > 0.0 && lh < 1.0) ? gLoH : 0.0;
-    o.n = (v * gnv + l * gnl) + h * gnh;
-    const d3 ghv = normalize_bwd(hv, n * gnh + l * glh);
-    o.v = n * gnv + ghv;
-    o.l = (n * gnl + h * glh) + ghv;
-    return {irr.x > 0.0 ? irr.x : 0.0, irr.y > 0.0 ? irr.y : 0.0, irr.z > 0.0 ? irr.z : 0.0};
+    const float gsL = gden * NoV, gsV = gden * NoL;
+    gNoL += gsL * NoL * (1.0f - a2) / sL;
+    gNoV += gsV * NoV * (1.0f - a2) / sV;
+    float ga2 = gsL * (1.0f - NoL * NoL) / (2.0f * sL) + gsV * (1.0f - NoV * NoV) / (2.0f * sV);
+    const f3 gf0 = mul(gF, 1.0f - p);
+    const float gp = (gF.x * (1.0f - f0.x) + gF.y * (1.0f - f0.y)) + gF.z * (1.0f - f0.z);
+    gLoH += gp * (-5.0f * pow4f(1.0f - LoH));
+    const float E = D / fD;
+    ga2 += gD * (1.0f / ((kPi * fD) * fD) - 2.0f * E * NoH * NoH);
+    const float gNoH = gD * (-2.0f * E * (2.0f * NoH * (a2 - 1.0f)));
+    ga += ga2 * 2.0f * a;
+    o.pr = ga * 2.0f * pr;
+    o.metallic = dot3(gf0, sub(base, f3{refl, refl, refl})) - dot3(gdiff, base);
+    o.base = add(mul(gf0, m), mul(gdiff, 1.0f - m));
+    const float gnv = (nv > 1e-5f && nv < 1.0f) ? gNoV : 0.0f;
+    const float gnl = (nl > 0.0f && nl < 1.0f) ? gNoL : 0.0f;
+    const float gnh = (nh > 0.0f && nh < 1.0f) ? gNoH : 0.0f;
+    const float glh = (lh > 0.0f && lh < 1.0f) ? gLoH : 0.0f;
+    o.n = add(add(mul(v, gnv), mul(l, gnl)), mul(h, gnh));
+    const f3 ghv = normalize_bwd(hv, add(mul(n, gnh), mul(l, glh)));
+    o.v = add(mul(n, gnv), ghv);
+    o.l = add(add(mul(n, gnl), mul(h, glh)), ghv);
+    return {irr.x > 0.0f ? irr.x : 0.0f, irr.y > 0.0f ? irr.y : 0.0f, irr.z > 0.0f ? irr.z : 0.0f};
 }
 
-// ---- textures in FP64 with the forward's (FP32) LOD held fixed ----
-struct Texel4 {
-    double c[4];
+// ---- textures with the forward's LOD held fixed ----
+// A sample and its derivatives with respect to uv, per channel, so one fetch
+// of the texels serves the value and the reverse.
+struct TexSample {
+    float val[4], du[4], dv[4];
 };
-LH_DEV Texel4 texel64(const LhTexture& t, uint32_t level, int32_t i, int32_t j) {
-    const int32_t w = int32_t(max(1u, t.width >> level)), h = int32_t(max(1u, t.height >> level));
+LH_DEV f4 texel(const LhTexture& t, uint32_t level, int32_t w, int32_t h, int32_t i, int32_t j) {
     i %= w;
     if (i < 0) i += w;
     j %= h;
     if (j < 0) j += h;
     const uint32_t q = ptr<uint32_t>(t.texels)[size_t(t.level_offset[level]) + size_t(j) * w + i];
-    return {{double(q & 255u) / 255.0, double((q >> 8) & 255u) / 255.0, double((q >> 16) & 255u) / 255.0,
-             double(q >> 24) / 255.0}};
+    return {float(q & 255u) / 255.0f, float((q >> 8) & 255u) / 255.0f, float((q >> 16) & 255u) / 255.0f,
+            float(q >> 24) / 255.0f};
 }
-// Bilinear value (val) and d/d(uv) contracted with g, at one level.
-LH_DEV void bilinear64(const LhTexture& t, uint32_t level, d2 uv, const double* g, double* val, d2& guv) {
-    const double w = double(max(1u, t.width >> level)), h = double(max(1u, t.height >> level));
-    const double u = uv.x * w - 0.5, vv = uv.y * h - 0.5;
+// Adds weight times the bilinear sample at one level (and its uv derivatives).
+// The texel cell and its fractions come from FP64 coordinates: FP32 would
+// pick a neighbouring cell for uvs within rounding of a texel boundary.
+LH_DEV void bilinear_acc(const LhTexture& t, uint32_t level, d2 uv, float weight, TexSample& o) {
+    const int32_t w = int32_t(max(1u, t.width >> level)), h = int32_t(max(1u, t.height >> level));
+    const double u = uv.x * double(w) - 0.5, vv = uv.y * double(h) - 0.5;
     const double fu = floor(u), fv = floor(vv);
-    const double a = u - fu, b = vv - fv;
+    const float a = float(u - fu), b = float(vv - fv);
     const int32_t i = int32_t(fu), j = int32_t(fv);
-    const Texel4 t00 = texel64(t, level, i, j), t10 = texel64(t, level, i + 1, j);
-    const Texel4 t01 = texel64(t, level, i, j + 1), t11 = texel64(t, level, i + 1, j + 1);
-    double ga = 0.0, gb = 0.0;
+    const f4 t00 = texel(t, level, w, h, i, j), t10 = texel(t, level, w, h, i + 1, j);
+    const f4 t01 = texel(t, level, w, h, i, j + 1), t11 = texel(t, level, w, h, i + 1, j + 1);
+    const float c00[4] = {t00.x, t00.y, t00.z, t00.w}, c10[4] = {t10.x, t10.y, t10.z, t10.w};
+    const float c01[4] = {t01.x, t01.y, t01.z, t01.w}, c11[4] = {t11.x, t11.y, t11.z, t11.w};
     for (int k = 0; k < 4; k++) {
-        const double top = t00.c[k] * (1.0 - a) + t10.c[k] * a, bot = t01.c[k] * (1.0 - a) + t11.c[k] * a;
-        val[k] = top * (1.0 - b) + bot * b;
-        ga += g[k] * ((1.0 - b) * (t10.c[k] - t00.c[k]) + b * (t11.c[k] - t01.c[k]));
-        gb += g[k] * (bot - top);
+        const float top = c00[k] * (1.0f - a) + c10[k] * a, bot = c01[k] * (1.0f - a) + c11[k] * a;
+        o.val[k] += weight * (top * (1.0f - b) + bot * b);
+        o.du[k] += weight * float(w) * ((1.0f - b) * (c10[k] - c00[k]) + b * (c11[k] - c01[k]));
+        o.dv[k] += weight * float(h) * (bot - top);
     }
-    guv = {ga * w, gb * h};
 }
-// Trilinear sample with the frozen LOD; the uv gradient is for the upstream g.
-// Unused index: white, no gradient.
-LH_DEV void sample64(const lh_scene& s, uint32_t index, const Lod& lod, d2 uv, const double* g, double* val, d2& guv) {
-    guv = {0.0, 0.0};
+// Trilinear sample with the frozen LOD.  Unused index: white, no gradient.
+LH_DEV TexSample sample_d(const lh_scene& s, uint32_t index, const Lod& lod, d2 uv) {
+    TexSample o;
+    for (int k = 0; k < 4; k++) o.val[k] = o.du[k] = o.dv[k] = 0.0f;
     if (!has_texture(s, index)) {
-        for (int k = 0; k < 4; k++) val[k] = 1.0;
-        return;
+        for (int k = 0; k < 4; k++) o.val[k] = 1.0f;
+        return o;
     }
     const LhTexture& t = s.textures[index];
-    bilinear64(t, lod.l0, uv, g, val, guv);
-    if (lod.f == 0.0f) return;
-    double v1[4];
-    d2 g1;
-    bilinear64(t, lod.l1, uv, g, v1, g1);
-    const double f = lod.f;
-    for (int k = 0; k < 4; k++) val[k] = val[k] * (1.0 - f) + v1[k] * f;
-    guv = {guv.x * (1.0 - f) + g1.x * f, guv.y * (1.0 - f) + g1.y * f};
+    bilinear_acc(t, lod.l0, uv, 1.0f - lod.f, o);
+    if (lod.f != 0.0f) bilinear_acc(t, lod.l1, uv, lod.f, o);
+    return o;
 }
 
 struct LightGrad {
@@ -204,7 +210,7 @@ struct VertexHash {
     float* vals;     // [kHashSlots * 3]
 };
 
-LH_DEV void add_geometry(const lh_resolve_grads& gr, const VertexHash& h, uint64_t addr, d3 g) {
+LH_DEV void add_geometry(const lh_resolve_grads& gr, const VertexHash& h, uint64_t addr, f3 g) {
     if (!gr.geometry || addr < gr.geometry_base) return;
     const uint64_t k64 = (addr - gr.geometry_base) / 4;
     if (k64 + 3 > gr.geometry_bytes / 4 || k64 >= kHashEmpty) return;
@@ -215,9 +221,9 @@ LH_DEV void add_geometry(const lh_resolve_grads& gr, const VertexHash& h, uint64
         if (old == kHashEmpty || old == k) break;
         slot = (slot + 1) & (kHashSlots - 1);
     }
-    atomicAdd(h.vals + 3 * slot, float(g.x));
-    atomicAdd(h.vals + 3 * slot + 1, float(g.y));
-    atomicAdd(h.vals + 3 * slot + 2, float(g.z));
+    atomicAdd(h.vals + 3 * slot, g.x);
+    atomicAdd(h.vals + 3 * slot + 1, g.y);
+    atomicAdd(h.vals + 3 * slot + 2, g.z);
 }
 
 // Called by the whole wave after a tile: adds every merged vec3 to the arena
@@ -272,11 +278,14 @@ LH_DEV Frozen frozen(const lh_scene& s, const lh_view& v, const MeshInfo& mi, co
 
 // One pixel: scatters vertex gradients, returns its material-factor gradients
 // (mesh info in mesh_info, or ~0u when it contributes nothing) and adds its
-// light gradients to lg.  The function and its reverse are evaluated in FP64
-// from the FP32 inputs.
+// light gradients to lg.  The clip-space chain (barycentrics and their
+// reverse) is evaluated in FP64 from the FP32 inputs, because for small,
+// distant triangles it cancels ~(depth / size)^2 of the FP32 precision; the
+// BRDF, texture and normal parts are well conditioned and run in FP32.
+template <bool kRecorded>
 LH_DEV void pixel_bwd(const lh_scene& s, const lh_view& v, const lh_resolve_params& prm, const float* grad_radiance,
                       const lh_resolve_grads& gr, const VertexHash& hash, uint32_t x, uint32_t y, uint32_t& mesh_info,
-                      d3& g_bcf, double& g_rf, double& g_mf, LightGrad& lg) {
+                      f3& g_bcf, float& g_rf, float& g_mf, LightGrad& lg) {
     mesh_info = ~0u;
     const size_t o = size_t(y) * v.width + x;
     Surface sf;
@@ -289,13 +298,14 @@ LH_DEV void pixel_bwd(const lh_scene& s, const lh_view& v, const lh_resolve_para
         return;
     }
     Frozen fz;
-    if (prm.frozen) {  // recorded by the forward (lh_resolve_ex with the same buffer)
+    if constexpr (kRecorded) {  // recorded by the forward (lh_resolve_ex with the same buffer)
         const uint4 r = reinterpret_cast<const uint4*>(prm.frozen)[o];
         fz.flip = (r.w & 1u) ? -1.0f : 1.0f;
         fz.K = kPi * __uint_as_float(r.x);
         auto lod = [&](uint32_t index, uint32_t l0, uint32_t fbits) {
             if (!has_texture(s, index)) return Lod{0, 0, 0.0f};
-            return Lod{l0, min(l0 + 1, s.textures[index].levels - 1), __uint_as_float(fbits)};
+            const uint32_t top = s.textures[index].levels - 1;
+            return Lod{l0, l0 + 1 < top ? l0 + 1 : top, __uint_as_float(fbits)};
         };
         fz.lod_b = lod(mi.base_color_texture_index, (r.w >> 8) & 255u, r.y);
         fz.lod_m = lod(mi.metallic_roughness_texture_index, (r.w >> 16) & 255u, r.z);
@@ -305,93 +315,108 @@ LH_DEV void pixel_bwd(const lh_scene& s, const lh_view& v, const lh_resolve_para
         fz = frozen(s, v, mi, sf, ndc, win);
     }
     const float4 gin = reinterpret_cast<const float4*>(grad_radiance)[o];
-    const d3 g{gin.x, gin.y, gin.z};
-    // ---- forward in FP64 ----
-    const LhMat4& T = in.transform;
+    const f3 g{gin.x, gin.y, gin.z};
+    // ---- forward: clip-space chain in FP64 ----
     const LhMat4& PV = u.combined_perspective_view;
-    d3 pos[3], c[3], nraw[3], nn[3];
-    d2 uvs[3];
-    for (int j = 0; j < 3; j++) {
-        const d3 L = D3(sf.local[j]);
-        pos[j] = ((col3(T.c[0]) * L.x + col3(T.c[1]) * L.y) + col3(T.c[2]) * L.z) + col3(T.c[3]);
-        // clip (x, y, w) = rows 0, 1, 3 of PV [pos, 1]
-        auto row = [&](int r) {
-            auto e = [&](int k) {
-                const LhVec4& cc = PV.c[k];
-                return double(r == 0 ? cc.x : r == 1 ? cc.y : cc.w);
-            };
-            return ((e(0) * pos[j].x + e(1) * pos[j].y) + e(2) * pos[j].z) + e(3);
-        };
-        c[j] = {row(0), row(1), row(3)};
-        nraw[j] = D3(load_normal(mi, sf.vi[j]));
-        nn[j] = normalize(nraw[j]);
-        const f2 t = load_uv(mi, sf.vi[j]);
-        uvs[j] = {double(t.x) * double(mi.texture_scale.x), double(t.y) * double(mi.texture_scale.y)};
-    }
-    // perspective-correct barycentrics: lambda_j = D_j / sum D, D_j = q . (c_{j+1} x c_{j+2})
     const d3 q{double(x) / double(u.window_size.x) * 2.0 - 1.0, double(y) / double(u.window_size.y) * 2.0 - 1.0, 1.0};
-    double Dk[3];
-    for (int k = 0; k < 3; k++) Dk[k] = dot(q, cross(c[(k + 1) % 3], c[(k + 2) % 3]));
-    const double sumD = (Dk[0] + Dk[1]) + Dk[2];
-    const double lam[3] = {Dk[0] / sumD, Dk[1] / sumD, Dk[2] / sumD};
-    const d3 world = (pos[0] * lam[0] + pos[1] * lam[1]) + pos[2] * lam[2];
-    const d2 uv{((lam[0] * uvs[0].x + lam[1] * uvs[1].x) + lam[2] * uvs[2].x) + double(mi.texture_offset.x),
-                ((lam[0] * uvs[0].y + lam[1] * uvs[1].y) + lam[2] * uvs[2].y) + double(mi.texture_offset.y)};
-    const d3 to_cam = D3(u.camera_pos) - world;
-    const d3 V = normalize(to_cam);
-    const d3 nI = (nn[0] * lam[0] + nn[1] * lam[1]) + nn[2] * lam[2];
+    d3 c[3];
+    double sumD, lam[3];
+    f3 dpos[3], to_cam;  // world positions relative to vertex 0; camera - world
+    {
+        const LhMat4& T = in.transform;
+        d3 pos[3];
+        for (int j = 0; j < 3; j++) {
+            const d3 L = D3(sf.local[j]);
+            pos[j] = ((col3(T.c[0]) * L.x + col3(T.c[1]) * L.y) + col3(T.c[2]) * L.z) + col3(T.c[3]);
+            // clip (x, y, w) = rows 0, 1, 3 of PV [pos, 1]
+            auto row = [&](int r) {
+                auto e = [&](int k) {
+                    const LhVec4& cc = PV.c[k];
+                    return double(r == 0 ? cc.x : r == 1 ? cc.y : cc.w);
+                };
+                return ((e(0) * pos[j].x + e(1) * pos[j].y) + e(2) * pos[j].z) + e(3);
+            };
+            c[j] = {row(0), row(1), row(3)};
+        }
+        // perspective-correct barycentrics: lambda_j = D_j / sum D, D_j = q . (c_{j+1} x c_{j+2})
+        double Dk[3];
+        for (int k = 0; k < 3; k++) Dk[k] = dot(q, cross(c[(k + 1) % 3], c[(k + 2) % 3]));
+        sumD = (Dk[0] + Dk[1]) + Dk[2];
+        for (int k = 0; k < 3; k++) lam[k] = Dk[k] / sumD;
+        const d3 tc = D3(u.camera_pos) - ((pos[0] * lam[0] + pos[1] * lam[1]) + pos[2] * lam[2]);
+        to_cam = {float(tc.x), float(tc.y), float(tc.z)};
+        for (int j = 0; j < 3; j++) {
+            const d3 dp = pos[j] - pos[0];
+            dpos[j] = {float(dp.x), float(dp.y), float(dp.z)};
+        }
+    }
+    const float lamf[3] = {float(lam[0]), float(lam[1]), float(lam[2])};
+    d2 uv;  // FP64: it selects the texel cells
+    f2 duv[3];
+    {
+        d2 uvs[3];
+        for (int j = 0; j < 3; j++) {
+            const f2 t = load_uv(mi, sf.vi[j]);
+            uvs[j] = {double(t.x) * double(mi.texture_scale.x), double(t.y) * double(mi.texture_scale.y)};
+        }
+        uv = {((lam[0] * uvs[0].x + lam[1] * uvs[1].x) + lam[2] * uvs[2].x) + double(mi.texture_offset.x),
+              ((lam[0] * uvs[0].y + lam[1] * uvs[1].y) + lam[2] * uvs[2].y) + double(mi.texture_offset.y)};
+        for (int j = 0; j < 3; j++) duv[j] = {float(uvs[j].x - uvs[0].x), float(uvs[j].y - uvs[0].y)};
+    }
+    // ---- forward: shading in FP32 ----
+    f3 nraw[3], nn[3];
+    for (int j = 0; j < 3; j++) {
+        nraw[j] = load_normal(mi, sf.vi[j]);
+        nn[j] = normalize3(nraw[j]);
+    }
+    const f3 V = normalize3(to_cam);
+    const f3 nI = add(add(mul(nn[0], lamf[0]), mul(nn[1], lamf[1])), mul(nn[2], lamf[2]));
     const LhMat3& N = in.normal_transform;
-    const d3 m3 = (D3(N.c[0]) * nI.x + D3(N.c[1]) * nI.y) + D3(N.c[2]) * nI.z;
-    const d3 nrm = normalize(m3) * double(fz.flip);
-    const d3 bcf = D3(param_base_color(prm, mi, sf.mesh_info));
-    const double rf = param_roughness(prm, s, sf.mesh_info), mf = param_metallic(prm, s, sf.mesh_info);
-    const d3 sun_dir = D3(param_sun_dir(prm, u)), sun_int = D3(param_sun_intensity(prm, u));
-    // the texture samples are needed before their upstream gradients are known:
-    // take the values first (zero upstream), the uv gradients after the reverse
-    const double zero4[4] = {0.0, 0.0, 0.0, 0.0};
-    double bc[4], mr[4];
-    d2 unused;
-    sample64(s, mi.base_color_texture_index, fz.lod_b, uv, zero4, bc, unused);
-    sample64(s, mi.metallic_roughness_texture_index, fz.lod_m, uv, zero4, mr, unused);
-    const d3 base = d3{bc[0], bc[1], bc[2]} * bcf;
-    const double roughness = mr[1] * rf, metallic = mr[2] * mf;
-    const double shv = nrm.y * 0.025 + 0.025;  // SH ambient: constant and y bands
-    const double K = fz.K;
+    const f3 m3 = add(add(mul(mk3(N.c[0]), nI.x), mul(mk3(N.c[1]), nI.y)), mul(mk3(N.c[2]), nI.z));
+    const f3 nrm = mul(normalize3(m3), fz.flip);
+    const f3 bcf = param_base_color(prm, mi, sf.mesh_info);
+    const float rf = param_roughness(prm, s, sf.mesh_info), mf = param_metallic(prm, s, sf.mesh_info);
+    const f3 sun_dir = param_sun_dir(prm, u), sun_int = param_sun_intensity(prm, u);
+    const TexSample tb = sample_d(s, mi.base_color_texture_index, fz.lod_b, uv);
+    const TexSample tm = sample_d(s, mi.metallic_roughness_texture_index, fz.lod_m, uv);
+    const f3 bc{tb.val[0], tb.val[1], tb.val[2]};
+    const f3 base = mul(bc, bcf);
+    const float roughness = tm.val[1] * rf, metallic = tm.val[2] * mf;
+    const float shv = nrm.y * 0.025f + 0.025f;  // SH ambient: constant and y bands
+    const float K = fz.K;
     // ---- reverse ----
     // out = B * K * I + base * (1 - metallic) * shv;  B needs the upstream gB = g * I * K
     BrdfGrad bg;
-    const d3 B = brdf_bwd(V, sun_dir, nrm, roughness, metallic, base, g * sun_int * K, bg);
-    lg.sun_intensity = lg.sun_intensity + g * B * K;
-    lg.sun_dir = lg.sun_dir + bg.l;
-    const d3 gbase = g * ((1.0 - metallic) * shv) + bg.base;
-    const double gmet = -dot(g, base) * shv + bg.metallic;
-    d3 gnrm = bg.n;
-    gnrm.y += dot(g, base) * (1.0 - metallic) * 0.025;
+    const f3 B = brdf_bwd(V, sun_dir, nrm, roughness, metallic, base, mul(mul(g, sun_int), K), bg);
+    const f3 gI = mul(mul(g, B), K);
+    lg.sun_intensity = lg.sun_intensity + d3{gI.x, gI.y, gI.z};
+    lg.sun_dir = lg.sun_dir + d3{bg.l.x, bg.l.y, bg.l.z};
+    const f3 gbase = add(mul(g, (1.0f - metallic) * shv), bg.base);
+    const float gmet = -dot3(g, base) * shv + bg.metallic;
+    f3 gnrm = bg.n;
+    gnrm.y += dot3(g, base) * (1.0f - metallic) * 0.025f;
     mesh_info = sf.mesh_info;
-    g_bcf = gbase * d3{bc[0], bc[1], bc[2]};
-    g_rf = bg.pr * mr[1];
-    g_mf = gmet * mr[2];
-    const double gb4[4] = {gbase.x * bcf.x, gbase.y * bcf.y, gbase.z * bcf.z, 0.0};
-    const double gm4[4] = {0.0, bg.pr * rf, gmet * mf, 0.0};
-    double tmp[4];
-    d2 guv_b, guv_m;
-    sample64(s, mi.base_color_texture_index, fz.lod_b, uv, gb4, tmp, guv_b);
-    sample64(s, mi.metallic_roughness_texture_index, fz.lod_m, uv, gm4, tmp, guv_m);
-    const d2 guv{guv_b.x + guv_m.x, guv_b.y + guv_m.y};
+    g_bcf = mul(gbase, bc);
+    g_rf = bg.pr * tm.val[1];
+    g_mf = gmet * tm.val[2];
+    // uv gradient: the upstream of the sampled channels times their derivatives
+    const f3 gb3 = mul(gbase, bcf);
+    const float gr1 = bg.pr * rf, gm2 = gmet * mf;
+    const f2 guv{((gb3.x * tb.du[0] + gb3.y * tb.du[1]) + gb3.z * tb.du[2]) + (gr1 * tm.du[1] + gm2 * tm.du[2]),
+                 ((gb3.x * tb.dv[0] + gb3.y * tb.dv[1]) + gb3.z * tb.dv[2]) + (gr1 * tm.dv[1] + gm2 * tm.dv[2])};
     // normal chain: nrm = flip * normalize(N nI), nI = sum lambda_j normalize(nraw_j)
-    const d3 gm3 = normalize_bwd(m3, gnrm * double(fz.flip));
-    const d3 gnI{dot(D3(N.c[0]), gm3), dot(D3(N.c[1]), gm3), dot(D3(N.c[2]), gm3)};
+    const f3 gm3 = normalize_bwd(m3, mul(gnrm, fz.flip));
+    const f3 gnI{dot3(mk3(N.c[0]), gm3), dot3(mk3(N.c[1]), gm3), dot3(mk3(N.c[2]), gm3)};
     // world position through the view vector (shadow and LOD are fixed)
-    const d3 gworld = -normalize_bwd(to_cam, bg.v);
-    // positions relative to vertex 0: lambda sums to 1, so a common shift of
-    // the lambda gradients cancels and the relative form is better conditioned
+    const f3 gworld = neg(normalize_bwd(to_cam, bg.v));
+    // every lambda-weighted quantity relative to vertex 0: lambda sums to 1, so
+    // a common shift of the lambda gradients cancels and the relative form
+    // keeps their differences exact
     double glam[3];
-    d3 gpos[3];
     for (int j = 0; j < 3; j++) {
-        glam[j] = (dot(gworld, pos[j] - pos[0]) + dot(gnI, nn[j])) +
-                  (guv.x * (uvs[j].x - uvs[0].x) + guv.y * (uvs[j].y - uvs[0].y));
-        gpos[j] = gworld * lam[j];
-        add_geometry(gr, hash, mi.normals + 12ull * sf.vi[j], normalize_bwd(nraw[j], gnI * lam[j]));
+        glam[j] = double(dot3(gworld, dpos[j])) + double(dot3(gnI, sub(nn[j], nn[0]))) +
+                  double(guv.x * duv[j].x + guv.y * duv[j].y);
+        add_geometry(gr, hash, mi.normals + 12ull * sf.vi[j], normalize_bwd(nraw[j], mul(gnI, lamf[j])));
     }
     const double lg_dot = (glam[0] * lam[0] + glam[1] * lam[1]) + glam[2] * lam[2];
     d3 gc[3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
@@ -401,17 +426,19 @@ LH_DEV void pixel_bwd(const lh_scene& s, const lh_view& v, const lh_resolve_para
         gc[a] = gc[a] + cross(c[b], q) * gD;
         gc[b] = gc[b] + cross(q, c[a]) * gD;
     }
+    const LhMat4& T = in.transform;
     for (int j = 0; j < 3; j++) {
+        d3 gpos = d3{gworld.x, gworld.y, gworld.z} * lam[j];
         // clip (x, y, w) <- pos: d clip_r / d pos_k = PV.c[k][r]
         for (int k = 0; k < 3; k++) {
             const LhVec4& cc = PV.c[k];
             const double d = (double(cc.x) * gc[j].x + double(cc.y) * gc[j].y) + double(cc.w) * gc[j].z;
-            if (k == 0) gpos[j].x += d;
-            if (k == 1) gpos[j].y += d;
-            if (k == 2) gpos[j].z += d;
+            if (k == 0) gpos.x += d;
+            if (k == 1) gpos.y += d;
+            if (k == 2) gpos.z += d;
         }
-        const d3 gL{dot(col3(T.c[0]), gpos[j]), dot(col3(T.c[1]), gpos[j]), dot(col3(T.c[2]), gpos[j])};
-        add_geometry(gr, hash, mi.positions + 12ull * sf.vi[j], gL);
+        const d3 gL{dot(col3(T.c[0]), gpos), dot(col3(T.c[1]), gpos), dot(col3(T.c[2]), gpos)};
+        add_geometry(gr, hash, mi.positions + 12ull * sf.vi[j], f3{float(gL.x), float(gL.y), float(gL.z)});
     }
 }
 
@@ -426,6 +453,7 @@ LH_DEV void pixel_bwd(const lh_scene& s, const lh_view& v, const lh_resolve_para
 #define LH_BACKWARD_OCCUPANCY
 #endif
 
+template <bool kRecorded>
 __global__ void __launch_bounds__(64) LH_BACKWARD_OCCUPANCY k_resolve_backward(lh_scene s, OneView vs, lh_resolve_params prm,
                                                          const float* grad_radiance, lh_resolve_grads gr,
                                                          double* partials) {
@@ -443,10 +471,10 @@ __global__ void __launch_bounds__(64) LH_BACKWARD_OCCUPANCY k_resolve_backward(l
     for (uint32_t t = blockIdx.x; t < tiles; t += gridDim.x) {
         const uint32_t x = (t % tiles_x) * 8 + (threadIdx.x & 7), y = (t / tiles_x) * 8 + (threadIdx.x >> 3);
         uint32_t mesh_info = ~0u;
-        d3 g_bcf{0, 0, 0};
-        double g_rf = 0.0, g_mf = 0.0;
+        f3 g_bcf{0, 0, 0};
+        float g_rf = 0.0f, g_mf = 0.0f;
         if (x < v.width && y < v.height)
-            pixel_bwd(s, v, prm, grad_radiance, gr, hash, x, y, mesh_info, g_bcf, g_rf, g_mf, lg);
+            pixel_bwd<kRecorded>(s, v, prm, grad_radiance, gr, hash, x, y, mesh_info, g_bcf, g_rf, g_mf, lg);
         if (gr.geometry) flush_geometry(gr, hash);
         // material factors: one add per distinct mesh info of the wave
         uint64_t pending = ballot(mesh_info != ~0u);
@@ -523,8 +551,12 @@ hipError_t resolve_backward(const lh_scene& s, const OneView& vs, const lh_resol
     // the C entry point checks its size against kBackwardScratchBytes)
     double* partials = static_cast<double*>(v.scratch);
     mark("resolve_backward", st);
-    hipLaunchKernelGGL(k_resolve_backward, dim3(blocks), dim3(64), 0, st, s, vs, params, grad_radiance, grads,
-                       partials);
+    if (params.frozen)
+        hipLaunchKernelGGL(k_resolve_backward<true>, dim3(blocks), dim3(64), 0, st, s, vs, params, grad_radiance,
+                           grads, partials);
+    else
+        hipLaunchKernelGGL(k_resolve_backward<false>, dim3(blocks), dim3(64), 0, st, s, vs, params, grad_radiance,
+                           grads, partials);
     mark("reduce_light", st);
     hipLaunchKernelGGL(k_reduce_light, dim3(1), dim3(256), 0, st, partials, blocks, grads);
     return hipGetLastError();
