@@ -44,6 +44,8 @@ def parse():
     ap.add_argument("--copy-probe", action="store_true",
                     help="run 256 MiB device-to-device copies (HBM-counter calibration for rocprofv3 --pmc)")
     ap.add_argument("--no-atomic-probe", action="store_true", help="skip the atomicMin ceiling measurement")
+    ap.add_argument("--quantized-steps", type=int, default=100,
+                    help="frames timed on the same workload in the reference's quantized vertex formats (0 = skip)")
     ap.add_argument("--hiz", action="store_true",
                     help="time the frame with two-phase occlusion culling (output is identical either way; the "
                          "other setting is timed too)")
@@ -252,6 +254,35 @@ def diff_bench(steps: int, tri_scale_pct: int = 18, side: int = 1024) -> dict:
             "covered_pixels": covered, "forward_ms": round(fwd, 4), "backward_ms": round(bwd - zero, 4),
             "gradient_zeroing_ms": round(zero, 4), "backward_over_forward": round((bwd - zero) / fwd, 3),
             "unsupported_pixels": unsupported}
+
+
+def quantized_bench(preset: int, cfg, cam, steps: int, warmup: int) -> dict:
+    """The same frame with the scene in the reference's vertex formats (u16
+    positions, i8 normals, u16 uvs, u16 remap indices): wall-clock ms per
+    frame over `steps` frames, like the headline number."""
+    import torch
+
+    from lighthugger_amd import abi
+    from lighthugger_amd.lib import Renderer
+    from lighthugger_amd.scene import HostScene
+
+    scene = HostScene.preset(preset, vertex_format=abi.VERTEX_FORMAT_QUANTIZED)
+    r = Renderer(scene, cfg)
+    r.set_camera(cam)
+    for _ in range(max(1, warmup)):
+        r.frame()
+    torch.cuda.synchronize()
+    if r.status()[abi.CNT_OVERFLOW]:
+        raise RuntimeError("capacity overflow in the quantized-vertex frame")
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        r.frame()
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1000.0 / steps
+    _, _, vis0 = r.view.prefix(0)
+    candidates = int(scene.instance_triangles()[vis0["index"].astype(np.int64)].sum())
+    return {"ms_per_frame": round(ms, 4), "gtri_per_s": round(candidates / (ms * 1e-3) / 1e9, 3),
+            "visible_candidate_triangles": candidates, "geometry_bytes": int(scene.geometry.nbytes)}
 
 
 def batch_profile(scene, n_cameras: int = 64, repeats: int = 3) -> dict:
@@ -467,6 +498,12 @@ def main():
         roofline = {"kernel": dominant, "bound": "hbm", "achieved": None, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                     "frac": None, "traffic": None, "avg_launch_ms": round(per_launch[dominant], 5)}
 
+    quantized = None
+    if args.quantized_steps > 0:
+        quantized = quantized_bench(preset, cfg, cam, args.quantized_steps, args.warmup)
+        quantized["geometry_bytes_fp32"] = int(scene.geometry.nbytes)
+        torch.cuda.empty_cache()
+
     batched = None
     if args.batch_steps > 0 and args.workload == "large":
         del r
@@ -520,6 +557,7 @@ def main():
             "candidates": int(st_hiz[abi.CNT_HIZ_CANDIDATES]) if cfg.hiz else None,
         },
         "cpu_baseline": cpu,
+        "quantized_vertices": quantized,
         "batched_views": batched,
         "differentiable_resolve": differentiable,
     }

@@ -26,6 +26,17 @@ extern "C" {
 
 typedef struct lh_scene_host lh_scene_host; /* opaque */
 
+/* Vertex formats the producer writes.
+ * FP32: float3 positions and normals, float2 uvs, u32 remap indices
+ *       (MESH_INFO_FLAGS_FP32_VERTICES | MESH_INFO_FLAGS_32_BIT_INDICES).
+ * QUANTIZED: the reference's formats (src/shaders/common/buffer_references.glsl:24-50):
+ *       u16x3 positions on a per-mesh grid (its origin and step are folded
+ *       into every instance transform), i8x3 normals (n * 127), u16x2 uvs (the
+ *       grid folded into the mesh info's texture_scale / texture_offset), u16
+ *       remap indices when the mesh has at most 65,536 vertices, else u32. */
+#define LH_VERTEX_FORMAT_FP32 0
+#define LH_VERTEX_FORMAT_QUANTIZED 1
+
 /* Synthetic workloads. */
 #define LH_PRESET_SMALL 1  /* 64 instances of one 8,192-triangle sphere, 4 constant materials */
 #define LH_PRESET_LARGE 2  /* 2,000 instances of 50 meshes (5k-500k tris), 64 textured materials */
@@ -37,6 +48,7 @@ typedef struct lh_preset_config {
     uint64_t seed;            /* 0 = preset default (preset number) */
     uint32_t texture_size;    /* 0 = preset default (1024 for LARGE/STRESS) */
     uint32_t mesh_tri_scale_pct; /* 0 = 100; scales per-mesh triangle targets (CPU-sized tests) */
+    uint32_t vertex_format;   /* LH_VERTEX_FORMAT_* (0 = FP32) */
 } lh_preset_config;
 
 /* Camera of one view.  `target` is a point on the view axis; up is +Y. */
@@ -75,6 +87,9 @@ typedef struct lh_scene_arrays {
 
 /* ---- incremental builder (hand-built fixtures and presets) ---- */
 lh_scene_host* lh_scene_create(void);
+/* Selects the vertex format (LH_VERTEX_FORMAT_*) of the meshes added after
+ * it; only before the first mesh.  0 on success. */
+int32_t lh_scene_set_vertex_format(lh_scene_host* s, uint32_t format);
 /* Adds a mesh with FP32 positions/normals (3 floats each), uvs (2 floats) and
  * uint32 triangle indices; builds its meshlets.  Returns the mesh id or -1. */
 int32_t lh_scene_add_mesh(lh_scene_host* s, const float* positions, const float* normals,

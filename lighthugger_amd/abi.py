@@ -21,6 +21,7 @@ LH_KEY_TRI_BITS = 7
 LH_KEY_MESHLET_BITS = 25
 
 PRESET_SMALL, PRESET_LARGE, PRESET_STRESS = 1, 2, 3
+VERTEX_FORMAT_FP32, VERTEX_FORMAT_QUANTIZED = 0, 1  # LH_VERTEX_FORMAT_* (lh_scene.h)
 
 
 def _dt(fields, itemsize):
@@ -104,7 +105,7 @@ class Camera(C.Structure):
 
 class PresetConfig(C.Structure):
     _fields_ = [("preset", C.c_uint32), ("num_instances", C.c_uint32), ("seed", C.c_uint64),
-                ("texture_size", C.c_uint32), ("mesh_tri_scale_pct", C.c_uint32)]
+                ("texture_size", C.c_uint32), ("mesh_tri_scale_pct", C.c_uint32), ("vertex_format", C.c_uint32)]
 
 
 class SceneArrays(C.Structure):

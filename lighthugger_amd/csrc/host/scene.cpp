@@ -330,7 +330,44 @@ struct BuiltMesh {
     std::vector<uint8_t> micro;
     std::array<float, 4> sphere{};
     uint64_t off_pos = 0, off_nrm = 0, off_uv = 0, off_remap = 0, off_micro = 0, off_meshlets = 0;
+    // Quantized vertex format: src.pos holds u16 grid coordinates; a position
+    // is grid_lo + step * q (folded into each instance's transform), a uv is
+    // uv_lo + uv_step * q (folded into the mesh info's texture transform).
+    bool quantized = false, index16 = false;
+    float grid_lo[3] = {0, 0, 0}, step = 1.0f;
+    float uv_lo[2] = {0, 0}, uv_step[2] = {1, 1};
 };
+
+// Grid step of n values spanning [lo, hi] in u16.
+float u16_step(float lo, float hi) { return hi > lo ? (hi - lo) / 65535.0f : 1.0f; }
+uint16_t to_u16(float x, float lo, float step) {
+    return uint16_t(std::clamp(std::lround((x - lo) / step), 0l, 65535l));
+}
+
+// The reference's vertex format (KHR_mesh_quantization, read by
+// src/shaders/common/util.glsl:11-16 and buffer_references.glsl:38-50):
+// positions are u16x3 on a per-mesh grid whose origin and step go into each
+// instance's transform, so the kernels see float(u16) mesh-space coordinates.
+// One step for all three axes keeps directions (normal cones, bounding
+// spheres) valid under the grid mapping.  The positions are replaced by their
+// grid coordinates before meshlets and bounds are built, so every bound is in
+// the space the kernels transform.
+void quantize_positions(BuiltMesh& bm) {
+    MeshSource& m = bm.src;
+    float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (uint32_t v = 0; v < m.nv(); v++)
+        for (int k = 0; k < 3; k++) {
+            lo[k] = std::min(lo[k], m.pos[3 * v + k]);
+            hi[k] = std::max(hi[k], m.pos[3 * v + k]);
+        }
+    float step = 0.0f;
+    for (int k = 0; k < 3; k++) step = std::max(step, u16_step(lo[k], hi[k]));
+    for (uint32_t v = 0; v < m.nv(); v++)
+        for (int k = 0; k < 3; k++) m.pos[3 * v + k] = float(to_u16(m.pos[3 * v + k], lo[k], step));
+    bm.quantized = true;
+    for (int k = 0; k < 3; k++) bm.grid_lo[k] = lo[k];
+    bm.step = step;
+}
 
 // Meshlet bounds (meshoptimizer's published cluster-bounds method).
 void meshlet_bounds(const MeshSource& m, const uint32_t* remap, const uint8_t* micro, uint32_t ntri,
@@ -685,6 +722,7 @@ LhMat4 mul4(const LhMat4& a, const LhMat4& b) {
 // ------------------------------------------------------------------ scene
 
 struct lh_scene_host {
+    uint32_t vertex_format = LH_VERTEX_FORMAT_FP32;
     std::vector<std::unique_ptr<BuiltMesh>> meshes;
     std::vector<TextureData> textures;
     std::vector<MaterialSpec> materials;
@@ -716,6 +754,7 @@ int32_t add_mesh(lh_scene_host* s, MeshSource&& src) {
     if (s->finished || src.nt() == 0) return -1;
     auto bm = std::make_unique<BuiltMesh>();
     bm->src = std::move(src);
+    if (s->vertex_format == LH_VERTEX_FORMAT_QUANTIZED) quantize_positions(*bm);
     build_meshlets(*bm);
     if (bm->meshlets.size() >= (1u << 16)) return -1;  // MeshInfo.num_meshlets is u16
     s->meshes.push_back(std::move(bm));
@@ -727,6 +766,12 @@ int32_t add_mesh(lh_scene_host* s, MeshSource&& src) {
 extern "C" {
 
 lh_scene_host* lh_scene_create(void) { return new lh_scene_host(); }
+
+int32_t lh_scene_set_vertex_format(lh_scene_host* s, uint32_t format) {
+    if (!s || s->finished || !s->meshes.empty() || format > LH_VERTEX_FORMAT_QUANTIZED) return -1;
+    s->vertex_format = format;
+    return 0;
+}
 
 void lh_scene_destroy(lh_scene_host* s) { delete s; }
 
@@ -779,10 +824,39 @@ int32_t lh_scene_finish(lh_scene_host* s) {
     }
     // Geometry arena.
     for (auto& bm : s->meshes) {
-        bm->off_pos = append_bytes(s->geometry, bm->src.pos);
-        bm->off_nrm = append_bytes(s->geometry, bm->src.nrm);
-        bm->off_uv = append_bytes(s->geometry, bm->src.uv);
-        bm->off_remap = append_bytes(s->geometry, bm->remap);
+        const MeshSource& m = bm->src;
+        if (bm->quantized) {
+            std::vector<uint16_t> qp(m.pos.size()), qu(m.uv.size());
+            std::vector<int8_t> qn(m.nrm.size());
+            for (size_t i = 0; i < m.pos.size(); i++) qp[i] = uint16_t(m.pos[i]);  // grid coordinates already
+            for (size_t i = 0; i < m.nrm.size(); i++)
+                qn[i] = int8_t(std::lround(std::clamp(m.nrm[i], -1.0f, 1.0f) * 127.0f));
+            for (int k = 0; k < 2; k++) {
+                float lo = INFINITY, hi = -INFINITY;
+                for (uint32_t v = 0; v < m.nv(); v++) {
+                    lo = std::min(lo, m.uv[2 * v + k]);
+                    hi = std::max(hi, m.uv[2 * v + k]);
+                }
+                bm->uv_lo[k] = lo;
+                bm->uv_step[k] = u16_step(lo, hi);
+                for (uint32_t v = 0; v < m.nv(); v++) qu[2 * v + k] = to_u16(m.uv[2 * v + k], lo, bm->uv_step[k]);
+            }
+            bm->off_pos = append_bytes(s->geometry, qp);
+            bm->off_nrm = append_bytes(s->geometry, qn);
+            bm->off_uv = append_bytes(s->geometry, qu);
+            bm->index16 = m.nv() <= 65536;
+            if (bm->index16) {
+                std::vector<uint16_t> r16(bm->remap.begin(), bm->remap.end());
+                bm->off_remap = append_bytes(s->geometry, r16);
+            } else {
+                bm->off_remap = append_bytes(s->geometry, bm->remap);
+            }
+        } else {
+            bm->off_pos = append_bytes(s->geometry, m.pos);
+            bm->off_nrm = append_bytes(s->geometry, m.nrm);
+            bm->off_uv = append_bytes(s->geometry, m.uv);
+            bm->off_remap = append_bytes(s->geometry, bm->remap);
+        }
         bm->off_micro = append_bytes(s->geometry, bm->micro);
         bm->off_meshlets = append_bytes(s->geometry, bm->meshlets);
         s->unique_tris += bm->src.nt();
@@ -806,10 +880,17 @@ int32_t lh_scene_finish(lh_scene_host* s) {
             mi.micro_indices = bm.off_micro;
             mi.meshlets = bm.off_meshlets;
             mi.num_meshlets = uint16_t(bm.meshlets.size());
-            mi.flags = MESH_INFO_FLAGS_32_BIT_INDICES | MESH_INFO_FLAGS_FP32_VERTICES;
+            mi.flags = bm.quantized ? (bm.index16 ? 0u : MESH_INFO_FLAGS_32_BIT_INDICES)
+                                    : MESH_INFO_FLAGS_32_BIT_INDICES | MESH_INFO_FLAGS_FP32_VERTICES;
             mi.bounding_sphere = LhVec4{bm.sphere[0], bm.sphere[1], bm.sphere[2], bm.sphere[3]};
-            mi.texture_scale = ms.tscale;
-            mi.texture_offset = ms.toffset;
+            if (bm.quantized) {  // uv = (uv_lo + uv_step q) * scale + offset
+                mi.texture_scale = LhVec2{bm.uv_step[0] * ms.tscale.x, bm.uv_step[1] * ms.tscale.y};
+                mi.texture_offset = LhVec2{bm.uv_lo[0] * ms.tscale.x + ms.toffset.x,
+                                           bm.uv_lo[1] * ms.tscale.y + ms.toffset.y};
+            } else {
+                mi.texture_scale = ms.tscale;
+                mi.texture_offset = ms.toffset;
+            }
             mi.base_color_texture_index = uint16_t(ms.tex_base);
             mi.metallic_roughness_texture_index = uint16_t(ms.tex_mr);
             mi.normal_texture_index = uint16_t(ms.tex_normal);
@@ -826,8 +907,17 @@ int32_t lh_scene_finish(lh_scene_host* s) {
         const auto& is = s->instances[i];
         Instance in{};
         in.transform = is.transform;
+        const BuiltMesh& bm = *s->meshes[is.mesh];
+        if (bm.quantized) {  // transform * translate(grid_lo) * scale(step)
+            LhMat4 g{};
+            g.c[0] = LhVec4{bm.step, 0, 0, 0};
+            g.c[1] = LhVec4{0, bm.step, 0, 0};
+            g.c[2] = LhVec4{0, 0, bm.step, 0};
+            g.c[3] = LhVec4{bm.grid_lo[0], bm.grid_lo[1], bm.grid_lo[2], 1};
+            in.transform = mul4(is.transform, g);
+        }
         in.mesh_info_address = inst_mi[i];
-        in.normal_transform = normal_matrix(is.transform);
+        in.normal_transform = normal_matrix(in.transform);
         s->inst_out.push_back(in);
         s->meshlet_base.push_back(uint32_t(base));
         uint32_t n = s->mesh_infos[inst_mi[i]].num_meshlets;
@@ -1015,6 +1105,7 @@ lh_scene_host* lh_scene_build_preset(const lh_preset_config* cfg) {
     uint64_t seed = cfg->seed ? cfg->seed : cfg->preset;
     Rng rng(seed);
     auto s = std::make_unique<lh_scene_host>();
+    if (lh_scene_set_vertex_format(s.get(), cfg->vertex_format) != 0) return nullptr;
     const float tri_scale = (cfg->mesh_tri_scale_pct ? cfg->mesh_tri_scale_pct : 100) / 100.0f;
     if (cfg->preset == LH_PRESET_SMALL) {
         // One sphere of exactly 8,192 triangles (octahedron, level 5), 4 constant

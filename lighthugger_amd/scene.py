@@ -38,6 +38,8 @@ def scene_lib() -> C.CDLL:
                                               C.c_uint32, abi.Vec2, abi.Vec2]
         lib.lh_scene_add_instance.restype = C.c_int32
         lib.lh_scene_add_instance.argtypes = [C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p]
+        lib.lh_scene_set_vertex_format.restype = C.c_int32
+        lib.lh_scene_set_vertex_format.argtypes = [C.c_void_p, C.c_uint32]
         lib.lh_scene_finish.restype = C.c_int32
         lib.lh_scene_finish.argtypes = [C.c_void_p]
         lib.lh_scene_build_preset.restype = C.c_void_p
@@ -151,18 +153,21 @@ class HostScene:
         return len(self.meshlet_table)
 
     @classmethod
-    def preset(cls, preset: int, *, num_instances=0, seed=0, texture_size=0, tri_scale_pct=0) -> "HostScene":
-        cfg = abi.PresetConfig(preset, num_instances, seed, texture_size, tri_scale_pct)
+    def preset(cls, preset: int, *, num_instances=0, seed=0, texture_size=0, tri_scale_pct=0,
+               vertex_format=abi.VERTEX_FORMAT_FP32) -> "HostScene":
+        cfg = abi.PresetConfig(preset, num_instances, seed, texture_size, tri_scale_pct, vertex_format)
         return cls(scene_lib().lh_scene_build_preset(C.byref(cfg)))
 
     @classmethod
-    def build(cls, meshes, materials, instances, textures=()) -> "HostScene":
+    def build(cls, meshes, materials, instances, textures=(), vertex_format=abi.VERTEX_FORMAT_FP32) -> "HostScene":
         """meshes: [(positions[N,3], normals[N,3] | None, uvs[N,2] | None, indices[T,3])];
         materials: [Material]; instances: [(mesh, material, transform[4,4] column-major)];
-        textures: [uint8 RGBA array (H, W, 4)]."""
+        textures: [uint8 RGBA array (H, W, 4)]; vertex_format: abi.VERTEX_FORMAT_*."""
         lib = scene_lib()
         h = lib.lh_scene_create()
         try:
+            if lib.lh_scene_set_vertex_format(h, vertex_format) != 0:
+                raise ValueError(f"unknown vertex format {vertex_format}")
             for pos, nrm, uv, idx in meshes:
                 pos = np.ascontiguousarray(pos, np.float32)
                 idx = np.ascontiguousarray(idx, np.uint32)
@@ -209,13 +214,32 @@ class HostScene:
         off, n = int(mi["meshlets"]), int(mi["num_meshlets"])
         return np.frombuffer(self.geometry[off: off + n * 56].tobytes(), abi.MESHLET)
 
+    def _fp32(self, mesh_info: int) -> bool:
+        return bool(int(self.mesh_infos[mesh_info]["flags"]) & abi.MESH_INFO_FLAGS_FP32_VERTICES)
+
+    def _attr(self, mesh_info: int, field: str, count: int, comps: int, qtype) -> np.ndarray:
+        """A vertex attribute as the kernels read it, as float32: FP32 as
+        stored, quantized values converted (u16 grid coordinates, i8 normals)."""
+        off = int(self.mesh_infos[mesh_info][field])
+        if self._fp32(mesh_info):
+            return self.geometry[off: off + 4 * comps * count].view(np.float32).reshape(-1, comps)
+        n = np.dtype(qtype).itemsize * comps * count
+        return self.geometry[off: off + n].view(qtype).reshape(-1, comps).astype(np.float32)
+
     def positions(self, mesh_info: int, count: int) -> np.ndarray:
-        off = int(self.mesh_infos[mesh_info]["positions"])
-        return self.geometry[off: off + 12 * count].view(np.float32).reshape(-1, 3)
+        return self._attr(mesh_info, "positions", count, 3, np.uint16)
+
+    def normals(self, mesh_info: int, count: int) -> np.ndarray:
+        return self._attr(mesh_info, "normals", count, 3, np.int8)
+
+    def uvs(self, mesh_info: int, count: int) -> np.ndarray:
+        return self._attr(mesh_info, "uvs", count, 2, np.uint16)
 
     def remap(self, mesh_info: int, count: int) -> np.ndarray:
         off = int(self.mesh_infos[mesh_info]["indices"])
-        return self.geometry[off: off + 4 * count].view(np.uint32)
+        if int(self.mesh_infos[mesh_info]["flags"]) & abi.MESH_INFO_FLAGS_32_BIT_INDICES:
+            return self.geometry[off: off + 4 * count].view(np.uint32)
+        return self.geometry[off: off + 2 * count].view(np.uint16).astype(np.uint32)
 
     def micro(self, mesh_info: int, count: int) -> np.ndarray:
         off = int(self.mesh_infos[mesh_info]["micro_indices"])

@@ -22,12 +22,13 @@ pytestmark = pytest.mark.gpu
 REL_L2 = 1e-4
 
 
-def _setup(width=256, height=192, shadow=512, tri_scale_pct=10):
+def _setup(width=256, height=192, shadow=512, tri_scale_pct=10, vertex_format=abi.VERTEX_FORMAT_FP32):
     import torch
 
     from lighthugger_amd.lib import Renderer
 
-    scene = HostScene.preset(abi.PRESET_LARGE, texture_size=64, tri_scale_pct=tri_scale_pct)
+    scene = HostScene.preset(abi.PRESET_LARGE, texture_size=64, tri_scale_pct=tri_scale_pct,
+                             vertex_format=vertex_format)
     base = preset_camera(abi.PRESET_LARGE)
     cam = make_camera(base.position.tolist(), base.target.tolist(), width, height)
     cfg = ViewConfig(width, height, shadow, big_capacity=1 << 20)
@@ -145,3 +146,20 @@ def test_autograd_function_l2_loss():
         assert _rel(p["base_color"].grad.cpu().numpy(), g64["base_color"]) <= REL_L2
     finally:
         assert torch.equal(geom, d.geometry)
+
+
+def test_quantized_meshes_are_reported_unsupported():
+    """The adjoint covers FP32 vertices only: on a scene in the quantized
+    formats it flags LH_FLAG_BACKWARD_UNSUPPORTED and adds no gradient."""
+    import torch
+
+    from lighthugger_amd.diff import DifferentiableResolve
+
+    scene, r, sd, vd = _setup(128, 96, 256, tri_scale_pct=5, vertex_format=abi.VERTEX_FORMAT_QUANTIZED)
+    d = DifferentiableResolve(r)
+    p = d.parameters()
+    out = d.raw_backward(p, torch.ones(96, 128, 4, device="cuda"))
+    torch.cuda.synchronize()
+    assert d.unsupported()
+    for k, t in out.items():
+        assert not t.any().item(), k

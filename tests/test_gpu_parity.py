@@ -52,6 +52,20 @@ def test_large_frame_parity_full_size():
     assert rep["pass0"]["meshlets"] > 10000
 
 
+@pytest.mark.parametrize("preset", ["small", "large"])
+def test_quantized_vertex_frame_parity(preset):
+    """The reference's vertex formats (u16 positions on a per-mesh grid, i8
+    normals, u16 uvs, u16 remap indices) through every kernel, at the same
+    bars; the large case is the benchmark workload at full size."""
+    p = {"small": abi.PRESET_SMALL, "large": abi.PRESET_LARGE}[preset]
+    scene = HostScene.preset(p, vertex_format=abi.VERTEX_FORMAT_QUANTIZED)
+    assert not (scene.mesh_infos["flags"] & abi.MESH_INFO_FLAGS_FP32_VERTICES).any()
+    cam = preset_camera(p)
+    shadow = 1024 if preset == "small" else 2048
+    rep = _run(scene, cam, ViewConfig(cam.width, cam.height, shadow, big_capacity=1 << 21))
+    assert rep["covered_pixels"] > 10000
+
+
 def test_fixtures_match_oracle_and_golden():
     import hashlib
     import json

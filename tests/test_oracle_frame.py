@@ -156,3 +156,31 @@ def test_threaded_oracle_frame_is_identical():
     b = snapshot(vd)
     for k in ("visbuffer", "shadow", "radiance", "ldr", "misc", "counters"):
         assert np.array_equal(a[k], b[k]), k
+
+
+def test_quantized_frame_close_to_fp32_frame():
+    """The oracle reads the reference's quantized formats (u16 positions, i8
+    normals, u16 uvs, u16 remap indices): the quantized scene renders the
+    same triangles as the FP32 scene at nearly every pixel (positions moved
+    by at most half a grid step, ~1e-5 of the mesh size), and the shading
+    (through i8 normals) stays close.  A closeness check of the format, not a
+    parity bar: the GPU parity tests compare both formats with the oracle
+    bit for bit."""
+    cam = preset_camera(abi.PRESET_SMALL)
+    cfg = ViewConfig(cam.width // 2, cam.height // 2, 512)
+    cam = make_camera(cam.position.tolist(), cam.target.tolist(), cfg.width, cfg.height)
+    frames = []
+    for vf in (abi.VERTEX_FORMAT_FP32, abi.VERTEX_FORMAT_QUANTIZED):
+        scene = HostScene.preset(abi.PRESET_SMALL, vertex_format=vf)
+        frames.append(oracle_frame(scene, cfg, make_uniforms(cam, scene.num_instances)))
+    a, b = frames
+    clear = np.uint64(abi.LH_VISBUFFER_CLEAR)
+    ids_a, ids_b = a["visbuffer"] & np.uint64(0xFFFFFFFF), b["visbuffer"] & np.uint64(0xFFFFFFFF)
+    covered = a["visbuffer"] != clear
+    assert covered.sum() > 4000
+    same = (ids_a == ids_b) & covered
+    assert same.sum() >= 0.99 * covered.sum(), same.sum() / covered.sum()
+    ra, rb = a["radiance"][..., :3][same], b["radiance"][..., :3][same]
+    rel = np.abs(ra - rb) / np.maximum(np.abs(ra), 1e-3)
+    # i8 normals move the shading by ~1% (more in highlights); shadow edges flip a few pixels
+    assert np.median(rel) < 1e-2 and np.quantile(rel, 0.9) < 0.1

@@ -155,3 +155,75 @@ def test_invalid_inputs_rejected():
     pos = np.zeros((3, 3), np.float32)
     with pytest.raises(ValueError):
         HostScene.build([(pos, None, None, np.array([[0, 1, 5]], np.uint32))], [], [])
+
+
+def _world(scene, inst, local):
+    m = np.asarray(scene.instances["transform"][inst], np.float64).T  # row-major
+    return local.astype(np.float64) @ m[:3, :3].T + m[:3, 3]
+
+
+@pytest.mark.parametrize("preset", [abi.PRESET_SMALL, abi.PRESET_LARGE])
+def test_quantized_vertices_dequantize_to_the_fp32_scene(preset):
+    """LH_VERTEX_FORMAT_QUANTIZED (the reference's u16 / i8 / u16 formats):
+    the meshlets and vertex order come from connectivity alone, so they equal
+    the FP32 scene's; world positions through the folded instance transform
+    are within half a grid step of the FP32 ones, normals within half an i8
+    step, texture coordinates (through the folded texture transform) within
+    half a u16 step; meshes up to 65,536 vertices use u16 remap indices."""
+    kw = dict(tri_scale_pct=5, texture_size=16) if preset == abi.PRESET_LARGE else {}
+    a = HostScene.preset(preset, **kw)
+    b = HostScene.preset(preset, vertex_format=abi.VERTEX_FORMAT_QUANTIZED, **kw)
+    assert b.geometry.nbytes < a.geometry.nbytes
+    assert not (b.mesh_infos["flags"] & abi.MESH_INFO_FLAGS_FP32_VERTICES).any()
+    inst_of = {int(k): i for i, k in reversed(list(enumerate(a.instances["mesh_info_address"])))}
+    for mi in range(len(a.mesh_infos)):
+        ma, mb = a.meshlets(mi), b.meshlets(mi)
+        for f in ("index_offset", "index_count", "triangle_offset", "triangle_count"):
+            assert np.array_equal(ma[f], mb[f]), f
+        n_idx = int(ma["index_offset"][-1]) + int(ma["index_count"][-1])
+        ra, rb = a.remap(mi, n_idx), b.remap(mi, n_idx)
+        assert np.array_equal(ra, rb)
+        nv = int(ra.max()) + 1
+        assert bool(b.mesh_infos["flags"][mi] & abi.MESH_INFO_FLAGS_32_BIT_INDICES) == (nv > 65536)
+        q = b.positions(mi, nv)
+        assert q.min() >= 0 and q.max() <= 65535 and np.array_equal(q, np.round(q))
+        # the grid: origin at the mesh's minimum corner, one step for all axes
+        pa = a.positions(mi, nv).astype(np.float64)
+        lo = pa.min(axis=0)
+        grid = (pa.max(axis=0) - lo).max() / 65535.0
+        assert np.abs(q - np.round((pa - lo) / grid)).max() <= 1.0
+        i = inst_of[mi]
+        ma_, mb_ = (np.asarray(x.instances["transform"][i], np.float64) for x in (a, b))
+        step = np.linalg.norm(mb_[0, :3]) / np.linalg.norm(ma_[0, :3])  # column 0 = M * step
+        assert abs(step - grid) <= 1e-5 * grid
+        # world positions: half a step per axis, plus the FP32 rounding of both translations
+        err = np.abs(_world(a, i, a.positions(mi, nv)) - _world(b, i, q)).max()
+        fp = 2.0 ** -23 * (np.abs(ma_[3, :3]).max() + np.abs(mb_[3, :3]).max() + 1.0)
+        assert err <= 0.5 * step * np.linalg.norm(ma_[0, :3]) * np.sqrt(3) * (1 + 1e-3) + 2 * fp, (mi, err, step)
+        na = a.normals(mi, nv)
+        assert np.abs(b.normals(mi, nv) / 127.0 - na).max() <= 0.5 / 127 + 1e-6
+        # uv * scale + offset, as the resolve applies it
+        ua = a.uvs(mi, nv) * a.mesh_infos["texture_scale"][mi] + a.mesh_infos["texture_offset"][mi]
+        ub = b.uvs(mi, nv).astype(np.float64) * b.mesh_infos["texture_scale"][mi] + b.mesh_infos["texture_offset"][mi]
+        half = 0.5 * np.asarray(b.mesh_infos["texture_scale"][mi], np.float64)
+        assert (np.abs(ua - ub) <= half * (1 + 1e-3) + 1e-5 * (1 + np.abs(ua))).all(), mi
+        # normal matrix of the folded transform
+        m = np.asarray(b.instances["transform"][i], np.float64).T
+        n = np.asarray(b.instances["normal_transform"][i], np.float64).T
+        np.testing.assert_allclose(n, np.linalg.inv(m[:3, :3]).T, rtol=1e-4)
+
+
+def test_vertex_format_rejected_after_first_mesh():
+    from lighthugger_amd.scene import scene_lib
+
+    lib = scene_lib()
+    h = lib.lh_scene_create()
+    try:
+        assert lib.lh_scene_set_vertex_format(h, 2) != 0
+        assert lib.lh_scene_set_vertex_format(h, abi.VERTEX_FORMAT_QUANTIZED) == 0
+        pos = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0]], np.float32)
+        idx = np.array([[0, 1, 2]], np.uint32)
+        assert lib.lh_scene_add_mesh(h, pos.ctypes.data, None, None, 3, idx.ctypes.data, 1) == 0
+        assert lib.lh_scene_set_vertex_format(h, abi.VERTEX_FORMAT_FP32) != 0
+    finally:
+        lib.lh_scene_destroy(h)
