@@ -55,6 +55,8 @@ typedef void* lh_stream; /* hipStream_t */
 #define LH_CNT_HIZ_PHASE1 27     /* LH_VIEW_HIZ: main-view draws rasterized before the pyramid */
 #define LH_CNT_FRAGMENTS 32 /* + pass: covered samples (only with LH_VIEW_STATISTICS) */
 #define LH_CNT_ATOMICS 40   /* + pass: depth atomics issued (same; main view: samples with depth > 0) */
+#define LH_CNT_ALPHA_DRAWS 48 /* + pass: surviving meshlets of alpha-clip mesh infos (with LH_VIEW_HIZ: before
+                                 the occlusion test); zero skips the alpha-clip raster of the pass */
 #define LH_NUM_COUNTERS 64
 #define LH_OVERFLOW_INSTANCES 0x1u
 #define LH_OVERFLOW_DRAWS 0x2u
@@ -90,7 +92,13 @@ typedef struct lh_scene {
     uint32_t num_mesh_infos;
     uint32_t total_meshlets;
     uint32_t num_textures;
+    uint32_t flags;                         /* LH_SCENE_* */
 } lh_scene;
+/* The caller states that no mesh info has MESH_INFO_FLAGS_ALPHA_CLIP: the
+ * alpha-clip raster launches (double-sided, alpha test per sample) are
+ * skipped.  Without it they run and return at once when no alpha-clip
+ * meshlet survived the culls; with it, alpha-clip meshlets are not drawn. */
+#define LH_SCENE_OPAQUE_ONLY 0x1u
 
 typedef struct lh_view {
     const Uniforms* uniforms;   /* device copy of this view's uniforms */

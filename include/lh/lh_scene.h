@@ -104,6 +104,10 @@ int32_t lh_scene_add_material(lh_scene_host* s, LhVec3 base_color_factor, float 
                               float metallic_factor, uint32_t base_color_texture,
                               uint32_t metallic_roughness_texture, uint32_t normal_texture,
                               LhVec2 texture_scale, LhVec2 texture_offset);
+/* Marks a material alpha-clipped (MESH_INFO_FLAGS_ALPHA_CLIP on its mesh
+ * infos: double-sided raster, samples whose base-colour alpha is below 0.5
+ * discarded, thin-walled BRDF, shadow bias 2e-6).  0 on success. */
+int32_t lh_scene_set_material_alpha_clip(lh_scene_host* s, uint32_t material, uint32_t enabled);
 /* Adds an instance of (mesh, material) with a column-major 4x4 transform. */
 int32_t lh_scene_add_instance(lh_scene_host* s, uint32_t mesh, uint32_t material,
                               const LhMat4* transform);

@@ -12,6 +12,7 @@ import numpy as np
 MESH_INFO_FLAGS_32_BIT_INDICES = 0x01
 MESH_INFO_FLAGS_ALPHA_CLIP = 0x02
 MESH_INFO_FLAGS_FP32_VERTICES = 0x04
+SCENE_OPAQUE_ONLY = 0x1  # LH_SCENE_OPAQUE_ONLY
 UNUSED_TEXTURE_INDEX = 0xFFFF
 MAX_MESHLET_UNIQUE_VERTICES = 64
 MAX_MESHLET_TRIANGLES = 124
@@ -126,7 +127,7 @@ class LhScene(C.Structure):
         ("instances", C.c_uint64), ("mesh_infos", C.c_uint64), ("factors", C.c_uint64),
         ("instance_meshlet_base", C.c_uint64), ("meshlet_table", C.c_uint64), ("textures", C.c_uint64),
         ("num_instances", C.c_uint32), ("num_mesh_infos", C.c_uint32), ("total_meshlets", C.c_uint32),
-        ("num_textures", C.c_uint32),
+        ("num_textures", C.c_uint32), ("flags", C.c_uint32),
     ]
 
 
@@ -148,6 +149,7 @@ NUM_CASCADES = 4
 NUM_COUNTERS = 64
 CNT_DRAWS, CNT_BIG, CNT_CLIP, CNT_OVERFLOW, CNT_FLAGS = 0, 8, 16, 24, 25
 CNT_FRAGMENTS, CNT_ATOMICS = 32, 40  # + pass, with VIEW_STATISTICS
+CNT_ALPHA_DRAWS = 48  # + pass: surviving alpha-clip meshlets
 VIEW_STATISTICS = 0x1
 VIEW_HIZ = 0x2
 CNT_HIZ_CANDIDATES, CNT_HIZ_PHASE1 = 26, 27

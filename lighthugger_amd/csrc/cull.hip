@@ -222,7 +222,7 @@ __global__ void __launch_bounds__(256) k_cull_meshlets(lh_scene s, VS vs, uint32
     const uint32_t stride = gridDim.x * blockDim.x;
     for (uint32_t base = blockIdx.x * blockDim.x; base < total; base += stride) {
         uint32_t id = base + threadIdx.x;
-        bool survive = false, candidate = false;
+        bool survive = false, candidate = false, alpha = false;
         PrefixSumValue e{0, 0};
         uint32_t local = 0;
         if (id < total && prefix_search(sec, n, id, e)) {
@@ -230,7 +230,7 @@ __global__ void __launch_bounds__(256) k_cull_meshlets(lh_scene s, VS vs, uint32
             const MeshInfo& mi = mesh_info_at(in.mesh_info_address);
             local = id - (e.sum - mi.num_meshlets);
             const Meshlet& ml = ptr<Meshlet>(mi.meshlets)[local];
-            bool alpha = (mi.flags & MESH_INFO_FLAGS_ALPHA_CLIP) != 0;
+            alpha = (mi.flags & MESH_INFO_FLAGS_ALPHA_CLIP) != 0;
             if (pass == 0) {
                 survive = !cull_sphere(in, ml.bounding_sphere, u) && (alpha || !cull_cone(in, ml, u, false));
                 if (survive && hiz) {
@@ -253,6 +253,7 @@ __global__ void __launch_bounds__(256) k_cull_meshlets(lh_scene s, VS vs, uint32
                           (alpha || !cull_cone(in, ml, u, true));
             }
         }
+        wave_append(&v.counters[LH_CNT_ALPHA_DRAWS + pass], (survive || candidate) && alpha);
         uint32_t slot = wave_append(&v.counters[LH_CNT_DRAWS + pass], survive);
         if (survive) {
             if (slot < v.draw_capacity) {

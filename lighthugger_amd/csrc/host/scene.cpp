@@ -618,6 +618,7 @@ struct MaterialSpec {
     float roughness, metallic;
     uint32_t tex_base, tex_mr, tex_normal;
     LhVec2 tscale, toffset;
+    bool alpha_clip = false;  // MESH_INFO_FLAGS_ALPHA_CLIP: double-sided, base-colour alpha < 0.5 discarded
 };
 
 struct InstanceSpec {
@@ -810,6 +811,12 @@ int32_t lh_scene_add_material(lh_scene_host* s, LhVec3 base_color_factor, float 
     return int32_t(s->materials.size() - 1);
 }
 
+int32_t lh_scene_set_material_alpha_clip(lh_scene_host* s, uint32_t material, uint32_t enabled) {
+    if (!s || s->finished || material >= s->materials.size()) return -1;
+    s->materials[material].alpha_clip = enabled != 0;
+    return 0;
+}
+
 int32_t lh_scene_add_instance(lh_scene_host* s, uint32_t mesh, uint32_t material, const LhMat4* transform) {
     if (!s || s->finished || mesh >= s->meshes.size() || material >= s->materials.size() || !transform) return -1;
     s->instances.push_back(InstanceSpec{mesh, material, *transform});
@@ -882,6 +889,7 @@ int32_t lh_scene_finish(lh_scene_host* s) {
             mi.num_meshlets = uint16_t(bm.meshlets.size());
             mi.flags = bm.quantized ? (bm.index16 ? 0u : MESH_INFO_FLAGS_32_BIT_INDICES)
                                     : MESH_INFO_FLAGS_32_BIT_INDICES | MESH_INFO_FLAGS_FP32_VERTICES;
+            if (ms.alpha_clip) mi.flags |= MESH_INFO_FLAGS_ALPHA_CLIP;
             mi.bounding_sphere = LhVec4{bm.sphere[0], bm.sphere[1], bm.sphere[2], bm.sphere[3]};
             if (bm.quantized) {  // uv = (uv_lo + uv_step q) * scale + offset
                 mi.texture_scale = LhVec2{bm.uv_step[0] * ms.tscale.x, bm.uv_step[1] * ms.tscale.y};

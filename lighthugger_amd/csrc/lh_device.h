@@ -15,6 +15,7 @@
 #include "lh/lh.h"
 
 #define LH_DEV __device__ __forceinline__
+#define LH_HD __host__ __device__ __forceinline__
 
 // Occupancy targets (waves per SIMD) of the two register-heavy kernels; a
 // build may override them.  0 leaves the register budget to the compiler.
@@ -167,7 +168,7 @@ LH_DEV uint64_t* sector(const lh_view& v, uint32_t pass) {
 // how one launch covers every view of a batch with the same arithmetic.
 struct OneView {
     lh_view v;
-    uint32_t count() const { return 1; }
+    LH_HD uint32_t count() const { return 1; }
     LH_DEV const lh_view& get(uint32_t) const { return v; }
 };
 struct ViewTable {
@@ -176,7 +177,7 @@ struct ViewTable {
     // Per-batch plan arrays (device): flattened raster segments and resolve tiles.
     uint32_t* seg_offsets;        // [LH_NUM_PASSES * n + 1]
     const uint32_t* tile_offsets; // [n + 1] resolve tiles (8x8) per view, exclusive scan
-    uint32_t count() const { return n; }
+    LH_HD uint32_t count() const { return n; }
     LH_DEV const lh_view& get(uint32_t i) const { return views[i]; }
 };
 

@@ -29,8 +29,9 @@
 //                      the pass).
 //   k_raster_clip      clips queued triangles and queues the fan pieces.
 //   k_raster_big       one 256-thread block per queued triangle, 16x16 tiles.
-#include "lh_device.h"
 #include "kernels.h"
+#include "lh_device.h"
+#include "shading.h"
 
 using namespace lhd;
 
@@ -44,7 +45,7 @@ struct BigTri {
     int32_t Y[3];
     float z[3];
     uint32_t id;
-    uint16_t xlo, ylo, xhi, yhi;  // xlo bits 13..15: pass index within the set
+    uint16_t xlo, ylo, xhi, yhi;  // xlo bits 13..15: pass index within the set; ylo bit 15: alpha clip
 };
 static_assert(sizeof(BigTri) == LH_BIG_ENTRY_BYTES, "big queue entry");
 
@@ -71,6 +72,17 @@ struct PassSet {
     uint32_t phase2;        // main view, second occlusion phase: draws from LH_CNT_HIZ_PHASE1 on
 };
 
+template <int KIND>
+LH_DEV const LhMat4& view_matrix(const lh_view& v, uint32_t pass) {
+    return KIND == 0 ? v.uniforms->combined_perspective_view : v.misc->shadow_matrices[pass - 1];
+}
+
+// The alpha test of one covered sample (alpha-clip geometry only).
+template <int KIND>
+LH_DEV bool alpha_pass(const lh_scene& s, const lh_view& v, const Pass& p, uint32_t id, int32_t px, int32_t py) {
+    return shading::alpha_visible(s, view_matrix<KIND>(v, p.pass), p.W, p.H, id, px, py);
+}
+
 LH_DEV Pass make_pass(const lh_view& v, uint32_t pass) {
     Pass p;
     p.pass = pass;
@@ -89,7 +101,10 @@ LH_DEV Pass make_pass(const lh_view& v, uint32_t pass) {
 }
 
 // Queue layout inside lh_view.scratch: clip entries {draw << 7 | triangle,
-// pass index within the set}, then (256-byte aligned) the big-triangle entries.
+// pass index within the set | alpha clip << 31}, then (256-byte aligned) the
+// big-triangle entries.
+constexpr uint32_t kClipAlpha = 0x80000000u;
+constexpr uint16_t kBigAlpha = 0x8000u;
 struct Queues {
     uint2* clip;
     BigTri* big;
@@ -138,12 +153,14 @@ LH_DEV bool top_left(int32_t ax, int32_t ay, int32_t bx, int32_t by) {
     return dy < 0 || (dy == 0 && dx > 0);
 }
 
-// Culls by facing, orients to positive area and computes the pixel bbox.
+// Culls by facing (alpha-clip geometry is double-sided: only zero area is
+// culled, src/pipelines.cpp:22-25), orients to positive area and computes the
+// pixel bbox.
 template <int KIND>
 LH_DEV bool setup(SV& v0, SV& v1, SV& v2, uint32_t W, uint32_t H, int64_t& area, int32_t& xlo, int32_t& xhi,
-                  int32_t& ylo, int32_t& yhi, int32_t& extx, int32_t& exty) {
+                  int32_t& ylo, int32_t& yhi, int32_t& extx, int32_t& exty, bool two_sided = false) {
     area = edge64(v0.X, v0.Y, v1.X, v1.Y, v2.X, v2.Y);
-    if (KIND == 0 ? area >= 0 : area <= 0) return false;
+    if (two_sided ? area == 0 : (KIND == 0 ? area >= 0 : area <= 0)) return false;
     if (area < 0) {
         SV t = v1;
         v1 = v2;
@@ -208,9 +225,9 @@ LH_DEV bool fragment(const Pass& p, int32_t px, int32_t py, float z, uint32_t id
 // In-lane raster of a triangle whose vertex bbox is within small_extent
 // (<= 32 px): every edge value fits in int32 and the edge functions are
 // stepped exactly in integers.
-template <int KIND, bool ST>
-LH_DEV void raster_small(const Pass& p, SV v0, SV v1, SV v2, int32_t area, int32_t xlo, int32_t xhi, int32_t ylo,
-                         int32_t yhi, uint32_t id, Tally<ST>& tally) {
+template <int KIND, bool ST, bool ALPHA>
+LH_DEV void raster_small(const lh_scene& s, const lh_view& v, const Pass& p, SV v0, SV v1, SV v2, int32_t area,
+                         int32_t xlo, int32_t xhi, int32_t ylo, int32_t yhi, uint32_t id, Tally<ST>& tally) {
     const bool tl0 = top_left(v1.X, v1.Y, v2.X, v2.Y), tl1 = top_left(v2.X, v2.Y, v0.X, v0.Y),
                tl2 = top_left(v0.X, v0.Y, v1.X, v1.Y);
     const float inv_area = 1.0f / float(area);
@@ -225,7 +242,7 @@ LH_DEV void raster_small(const Pass& p, SV v0, SV v1, SV v2, int32_t area, int32
         int32_t e2 = (v1.X - v0.X) * (Py - v0.Y) - (v1.Y - v0.Y) * (Px0 - v0.X);
         for (int32_t px = xlo; px <= xhi; px++) {
             bool in = (e0 > 0 || (e0 == 0 && tl0)) && (e1 > 0 || (e1 == 0 && tl1)) && (e2 > 0 || (e2 == 0 && tl2));
-            if (in) {
+            if (in && (!ALPHA || alpha_pass<KIND>(s, v, p, id, px, py))) {
                 float l1 = float(e1) * inv_area, l2 = float(e2) * inv_area;
                 tally.hit(fragment<KIND>(p, px, py, (v0.z + l1 * d10) + l2 * d20, id));
             }
@@ -239,9 +256,9 @@ LH_DEV void raster_small(const Pass& p, SV v0, SV v1, SV v2, int32_t area, int32
 // Wave-cooperative raster of one medium triangle (vertex bbox within 64 px,
 // so every edge value fits in int32): the 64 lanes cover its pixel bbox in
 // 8x8 tiles.  All arguments are wave-uniform.
-template <int KIND, bool ST>
-LH_DEV void raster_wave(const Pass& p, SV v0, SV v1, SV v2, int32_t xlo, int32_t xhi, int32_t ylo, int32_t yhi,
-                        uint32_t id, uint32_t lane, Tally<ST>& tally) {
+template <int KIND, bool ST, bool ALPHA>
+LH_DEV void raster_wave(const lh_scene& s, const lh_view& v, const Pass& p, SV v0, SV v1, SV v2, int32_t xlo,
+                        int32_t xhi, int32_t ylo, int32_t yhi, uint32_t id, uint32_t lane, Tally<ST>& tally) {
     const int32_t area = (v1.X - v0.X) * (v2.Y - v0.Y) - (v1.Y - v0.Y) * (v2.X - v0.X);
     const bool tl0 = top_left(v1.X, v1.Y, v2.X, v2.Y), tl1 = top_left(v2.X, v2.Y, v0.X, v0.Y),
                tl2 = top_left(v0.X, v0.Y, v1.X, v1.Y);
@@ -256,7 +273,8 @@ LH_DEV void raster_wave(const Pass& p, SV v0, SV v1, SV v2, int32_t xlo, int32_t
             const int32_t e0 = (v2.X - v1.X) * (Py - v1.Y) - (v2.Y - v1.Y) * (Px - v1.X);
             const int32_t e1 = (v0.X - v2.X) * (Py - v2.Y) - (v0.Y - v2.Y) * (Px - v2.X);
             const int32_t e2 = (v1.X - v0.X) * (Py - v0.Y) - (v1.Y - v0.Y) * (Px - v0.X);
-            if ((e0 > 0 || (e0 == 0 && tl0)) && (e1 > 0 || (e1 == 0 && tl1)) && (e2 > 0 || (e2 == 0 && tl2))) {
+            if ((e0 > 0 || (e0 == 0 && tl0)) && (e1 > 0 || (e1 == 0 && tl1)) && (e2 > 0 || (e2 == 0 && tl2)) &&
+                (!ALPHA || alpha_pass<KIND>(s, v, p, id, px, py))) {
                 const float l1 = float(e1) * inv_area, l2 = float(e2) * inv_area;
                 tally.hit(fragment<KIND>(p, px, py, (v0.z + l1 * d10) + l2 * d20, id));
             }
@@ -268,13 +286,14 @@ LH_DEV int32_t readlane(int32_t v, uint32_t l) { return __builtin_amdgcn_readlan
 LH_DEV float readlane(float v, uint32_t l) { return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), int(l))); }
 
 LH_DEV void store_big(BigTri* q, const SV& v0, const SV& v1, const SV& v2, uint32_t id, int32_t xlo, int32_t xhi,
-                      int32_t ylo, int32_t yhi, uint32_t pi) {
+                      int32_t ylo, int32_t yhi, uint32_t pi, bool alpha) {
     BigTri b;
     b.X[0] = v0.X, b.X[1] = v1.X, b.X[2] = v2.X;
     b.Y[0] = v0.Y, b.Y[1] = v1.Y, b.Y[2] = v2.Y;
     b.z[0] = v0.z, b.z[1] = v1.z, b.z[2] = v2.z;
     b.id = id;
-    b.xlo = uint16_t(uint32_t(xlo) | (pi << 13)), b.ylo = uint16_t(ylo), b.xhi = uint16_t(xhi), b.yhi = uint16_t(yhi);
+    b.xlo = uint16_t(uint32_t(xlo) | (pi << 13)), b.ylo = uint16_t(uint32_t(ylo) | (alpha ? kBigAlpha : 0u));
+    b.xhi = uint16_t(xhi), b.yhi = uint16_t(yhi);
     uint4* dst = reinterpret_cast<uint4*>(q);
     const uint4* src = reinterpret_cast<const uint4*>(&b);
     dst[0] = src[0];
@@ -295,11 +314,6 @@ LH_DEV f3 load_position(const MeshInfo& mi, uint32_t v) {
     }
     const uint16_t* q = ptr<uint16_t>(mi.positions) + 3 * size_t(v);
     return {float(q[0]), float(q[1]), float(q[2])};
-}
-
-template <int KIND>
-LH_DEV const LhMat4& view_matrix(const lh_view& v, uint32_t pass) {
-    return KIND == 0 ? v.uniforms->combined_perspective_view : v.misc->shadow_matrices[pass - 1];
 }
 
 // Segment lookup of a flattened draw index g.  One view: the segments are the
@@ -382,12 +396,21 @@ struct Segments<ViewTable> {
     }
 };
 
-template <int KIND, class VS, bool ST>
+// ALPHA selects the draws of alpha-clip mesh infos (double-sided, alpha test
+// per sample); the opaque instantiation skips them.  The alpha instantiation
+// returns at once when the culls counted no alpha-clip draw in its passes.
+template <int KIND, class VS, bool ST, bool ALPHA>
 __global__ void __launch_bounds__(64) LH_RASTER_OCCUPANCY k_raster_meshlets(lh_scene s, VS vs, PassSet ps) {
     __shared__ int32_t sX[64], sY[64];
     __shared__ float sZ[64];
     __shared__ uint32_t sO[64];
     const uint32_t lane = threadIdx.x;
+    if constexpr (ALPHA) {
+        uint32_t n = 0;
+        for (uint32_t k = lane; k < vs.count() * ps.n; k += 64)
+            n += vs.get(k / ps.n).counters[LH_CNT_ALPHA_DRAWS + ps.pass0 + k % ps.n];
+        if (wave_sum_u32(n) == 0) return;
+    }
     const Segments<VS> seg(vs, ps);
     const uint32_t total = seg.total();
     Tally<ST> tally;
@@ -401,6 +424,7 @@ __global__ void __launch_bounds__(64) LH_RASTER_OCCUPANCY k_raster_meshlets(lh_s
         const MeshletReference ref = p.draws[d];
         const Instance& in = s.instances[ref.instance_index];
         const MeshInfo& mi = mesh_info_at(in.mesh_info_address);
+        if (((mi.flags & MESH_INFO_FLAGS_ALPHA_CLIP) != 0) != ALPHA) return;  // wave-uniform
         const Meshlet& ml = ptr<Meshlet>(mi.meshlets)[ref.meshlet_index];
         const uint32_t mid = s.instance_meshlet_base[ref.instance_index] + ref.meshlet_index;
         const uint32_t nvert = ml.index_count, ntri = ml.triangle_count;
@@ -449,7 +473,7 @@ __global__ void __launch_bounds__(64) LH_RASTER_OCCUPANCY k_raster_meshlets(lh_s
                         v0 = SV{sX[i0], sY[i0], sZ[i0]};
                         v1 = SV{sX[i1], sY[i1], sZ[i1]};
                         v2 = SV{sX[i2], sY[i2], sZ[i2]};
-                        if (setup<KIND>(v0, v1, v2, p.W, p.H, area, xlo, xhi, ylo, yhi, ex, ey)) {
+                        if (setup<KIND>(v0, v1, v2, p.W, p.H, area, xlo, xhi, ylo, yhi, ex, ey, ALPHA)) {
                             const int32_t e = max(ex, ey);
                             cls = e <= ps.small_extent ? 3u : e <= ps.medium_extent ? 4u : 2u;
                         }
@@ -460,26 +484,28 @@ __global__ void __launch_bounds__(64) LH_RASTER_OCCUPANCY k_raster_meshlets(lh_s
             uint32_t cslot = wave_append(clip_counter, cls == 1);
             if (cls == 1) {
                 if (cslot < v.clip_capacity)
-                    qs.clip[cslot] = make_uint2((d << LH_KEY_TRI_BITS) | t, pi);
+                    qs.clip[cslot] = make_uint2((d << LH_KEY_TRI_BITS) | t, pi | (ALPHA ? kClipAlpha : 0u));
                 else
                     atomicOr(&v.counters[LH_CNT_OVERFLOW], LH_OVERFLOW_CLIP);
             }
             uint32_t bslot = wave_append(big_counter, cls == 2);
             if (cls == 2) {
                 if (bslot < v.big_capacity)
-                    store_big(qs.big + bslot, v0, v1, v2, id, xlo, xhi, ylo, yhi, pi);
+                    store_big(qs.big + bslot, v0, v1, v2, id, xlo, xhi, ylo, yhi, pi, ALPHA);
                 else
                     atomicOr(&v.counters[LH_CNT_OVERFLOW], LH_OVERFLOW_BIG);
             }
-            if (cls == 3) raster_small<KIND>(p, v0, v1, v2, int32_t(area), xlo, xhi, ylo, yhi, id, tally);
+            if (cls == 3)
+                raster_small<KIND, ST, ALPHA>(s, v, p, v0, v1, v2, int32_t(area), xlo, xhi, ylo, yhi, id, tally);
             // Medium triangles: broadcast each lane's setup and raster it with the whole wave.
             for (uint64_t med = ballot(cls == 4); med; med &= med - 1) {
                 const uint32_t l = first_lane(med);
                 const SV a{readlane(v0.X, l), readlane(v0.Y, l), readlane(v0.z, l)};
                 const SV b{readlane(v1.X, l), readlane(v1.Y, l), readlane(v1.z, l)};
                 const SV c{readlane(v2.X, l), readlane(v2.Y, l), readlane(v2.z, l)};
-                raster_wave<KIND>(p, a, b, c, readlane(xlo, l), readlane(xhi, l), readlane(ylo, l), readlane(yhi, l),
-                                  (mid << LH_KEY_TRI_BITS) | (l + 64 * round), lane, tally);
+                raster_wave<KIND, ST, ALPHA>(s, v, p, a, b, c, readlane(xlo, l), readlane(xhi, l), readlane(ylo, l),
+                                             readlane(yhi, l), (mid << LH_KEY_TRI_BITS) | (l + 64 * round), lane,
+                                             tally);
             }
         }
         tally.flush(v, p.pass);
@@ -520,7 +546,8 @@ __global__ void __launch_bounds__(64) k_raster_clip(lh_scene s, VS vs, PassSet p
     uint32_t* big_counter = &v.counters[LH_CNT_BIG + ps.pass0];
     for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < count; q += gridDim.x * blockDim.x) {
         const uint2 e = qs.clip[q];
-        const uint32_t d = e.x >> LH_KEY_TRI_BITS, t = e.x & ((1u << LH_KEY_TRI_BITS) - 1), pi = e.y;
+        const uint32_t d = e.x >> LH_KEY_TRI_BITS, t = e.x & ((1u << LH_KEY_TRI_BITS) - 1), pi = e.y & ~kClipAlpha;
+        const bool alpha = (e.y & kClipAlpha) != 0;
         const Pass p = make_pass(v, ps.pass0 + pi);
         const LhMat4 M = view_matrix<KIND>(v, p.pass);
         const MeshletReference ref = p.draws[d];
@@ -557,10 +584,10 @@ __global__ void __launch_bounds__(64) k_raster_clip(lh_scene s, VS vs, PassSet p
             SV a = sv[0], b = sv[i], c = sv[i + 1];
             int64_t area;
             int32_t xlo, xhi, ylo, yhi, ex, ey;
-            if (!setup<KIND>(a, b, c, p.W, p.H, area, xlo, xhi, ylo, yhi, ex, ey)) continue;
+            if (!setup<KIND>(a, b, c, p.W, p.H, area, xlo, xhi, ylo, yhi, ex, ey, alpha)) continue;
             uint32_t slot = atomicAdd(big_counter, 1u);
             if (slot < v.big_capacity)
-                store_big(qs.big + slot, a, b, c, id, xlo, xhi, ylo, yhi, pi);
+                store_big(qs.big + slot, a, b, c, id, xlo, xhi, ylo, yhi, pi, alpha);
             else
                 atomicOr(&v.counters[LH_CNT_OVERFLOW], LH_OVERFLOW_BIG);
         }
@@ -569,22 +596,31 @@ __global__ void __launch_bounds__(64) k_raster_clip(lh_scene s, VS vs, PassSet p
 
 // View blockIdx.z; one 256-thread block per queued triangle; the bbox is
 // walked in 16x16 tiles and each pixel evaluates the int64 edge functions.
-template <int KIND, class VS, bool ST>
-__global__ void __launch_bounds__(256) k_raster_big(VS vs, PassSet ps) {
+// ALPHA selects the entries of alpha-clip geometry (alpha test per sample),
+// so the opaque instantiation keeps its small register footprint; the alpha
+// one returns at once when the view's culls counted no alpha-clip draw.
+template <int KIND, class VS, bool ST, bool ALPHA>
+__global__ void __launch_bounds__(256) k_raster_big(lh_scene s, VS vs, PassSet ps) {
     const lh_view& v = vs.get(blockIdx.z);
+    if constexpr (ALPHA) {
+        uint32_t n = 0;
+        for (uint32_t k = 0; k < ps.n; k++) n += v.counters[LH_CNT_ALPHA_DRAWS + ps.pass0 + k];
+        if (n == 0) return;
+    }
     const Queues qs = queues(v);
     const uint32_t count = min(v.counters[LH_CNT_BIG + ps.pass0], v.big_capacity);
     const int32_t tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
     Tally<ST> tally;
     for (uint32_t e = blockIdx.x; e < count; e += gridDim.x) {
         const BigTri b = qs.big[e];
+        if (((b.ylo & kBigAlpha) != 0) != ALPHA) continue;  // block-uniform
         const Pass p = make_pass(v, ps.pass0 + (uint32_t(b.xlo) >> 13));
         const int32_t X0 = b.X[0], X1 = b.X[1], X2 = b.X[2], Y0 = b.Y[0], Y1 = b.Y[1], Y2 = b.Y[2];
         const int64_t area = edge64(X0, Y0, X1, Y1, X2, Y2);
         const float inv_area = 1.0f / float(area);
         const float z0 = b.z[0], d10 = b.z[1] - b.z[0], d20 = b.z[2] - b.z[0];
         const bool tl0 = top_left(X1, Y1, X2, Y2), tl1 = top_left(X2, Y2, X0, Y0), tl2 = top_left(X0, Y0, X1, Y1);
-        const int32_t xlo = b.xlo & 0x1FFF, xhi = b.xhi, ylo = b.ylo, yhi = b.yhi;
+        const int32_t xlo = b.xlo & 0x1FFF, xhi = b.xhi, ylo = b.ylo & ~kBigAlpha, yhi = b.yhi;
         for (int32_t by = ylo; by <= yhi; by += 16) {
             for (int32_t bx = xlo; bx <= xhi; bx += 16) {
                 const int32_t px = bx + tx, py = by + ty;
@@ -596,6 +632,7 @@ __global__ void __launch_bounds__(256) k_raster_big(VS vs, PassSet ps) {
                 if (!(e0 > 0 || (e0 == 0 && tl0))) continue;
                 if (!(e1 > 0 || (e1 == 0 && tl1))) continue;
                 if (!(e2 > 0 || (e2 == 0 && tl2))) continue;
+                if (ALPHA && !alpha_pass<KIND>(s, v, p, b.id, px, py)) continue;
                 const float l1 = float(e1) * inv_area, l2 = float(e2) * inv_area;
                 tally.hit(fragment<KIND>(p, px, py, (z0 + l1 * d10) + l2 * d20, b.id));
             }
@@ -636,11 +673,20 @@ void launch_kernels(const lh_scene& s, const VS& vs, const PassSet& ps, uint32_t
                     uint32_t big_blocks, hipStream_t st) {
     const uint32_t nv = vs.count();
     mark(KIND == 0 ? "raster_meshlets_view" : "raster_meshlets_shadow", st);
-    hipLaunchKernelGGL((k_raster_meshlets<KIND, VS, ST>), dim3(meshlet_blocks), dim3(64), 0, st, s, vs, ps);
+    hipLaunchKernelGGL((k_raster_meshlets<KIND, VS, ST, false>), dim3(meshlet_blocks), dim3(64), 0, st, s, vs, ps);
+    const bool alpha = !(s.flags & LH_SCENE_OPAQUE_ONLY);
+    if (alpha) {
+        mark(KIND == 0 ? "raster_alpha_view" : "raster_alpha_shadow", st);
+        hipLaunchKernelGGL((k_raster_meshlets<KIND, VS, ST, true>), dim3(meshlet_blocks), dim3(64), 0, st, s, vs, ps);
+    }
     mark(KIND == 0 ? "raster_clip_view" : "raster_clip_shadow", st);
     hipLaunchKernelGGL((k_raster_clip<KIND, VS>), dim3(clip_blocks, 1, nv), dim3(64), 0, st, s, vs, ps);
     mark(KIND == 0 ? "raster_big_view" : "raster_big_shadow", st);
-    hipLaunchKernelGGL((k_raster_big<KIND, VS, ST>), dim3(big_blocks, 1, nv), dim3(256), 0, st, vs, ps);
+    hipLaunchKernelGGL((k_raster_big<KIND, VS, ST, false>), dim3(big_blocks, 1, nv), dim3(256), 0, st, s, vs, ps);
+    if (alpha) {
+        mark(KIND == 0 ? "raster_big_alpha_view" : "raster_big_alpha_shadow", st);
+        hipLaunchKernelGGL((k_raster_big<KIND, VS, ST, true>), dim3(big_blocks, 1, nv), dim3(256), 0, st, s, vs, ps);
+    }
 }
 
 // Statistics are a single-view feature (LH_VIEW_STATISTICS).

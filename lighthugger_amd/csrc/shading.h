@@ -265,6 +265,45 @@ LH_DEV bool decode(const lh_scene& s, const lh_view& v, uint64_t key, Surface& s
     return true;
 }
 
+// Alpha test of the alpha-clip pipelines (visbuffer_alpha_clip_pixel and
+// shadowmap_alpha_clipped_pixel, src/shaders/rasterization.glsl:84-97 and
+// :130-145): the base-colour texture's alpha at the pixel centre, discarded
+// below 0.5.  The uv is interpolated perspective-correctly and its LOD taken
+// from the analytic derivatives (CalcFullBary, as the resolve does) in place
+// of the hardware's 2x2-quad differences.  `m` is the pass's view-projection
+// matrix, W x H its target; the triangle is the one the key id names.  A
+// triangle with a vertex at or behind w = 0, or a non-finite uv, passes.
+LH_DEV bool alpha_visible(const lh_scene& s, const LhMat4& m, uint32_t W, uint32_t H, uint32_t id, int32_t px,
+                          int32_t py) {
+    const MeshletReference ref = s.meshlet_table[id >> LH_KEY_TRI_BITS];
+    const Instance& in = s.instances[ref.instance_index];
+    const MeshInfo& mi = mesh_info_at(in.mesh_info_address);
+    if (!has_texture(s, mi.base_color_texture_index)) return true;  // white, alpha 1
+    const Meshlet& ml = ptr<Meshlet>(mi.meshlets)[ref.meshlet_index];
+    const uint8_t* micro = ptr<uint8_t>(mi.micro_indices) + ml.triangle_offset + 3 * (id & ((1u << LH_KEY_TRI_BITS) - 1));
+    const f2 ts{mi.texture_scale.x, mi.texture_scale.y};
+    f4 clip[3];
+    f2 uvs[3];
+    for (int j = 0; j < 3; j++) {
+        const uint32_t vi = load_index(mi, ml.index_offset + micro[j]);
+        const f3 w = mat4_point(in.transform, load_position(mi, vi));
+        clip[j] = mat4_mul_vec4(m, f4{w.x, w.y, w.z, 1.0f});
+        if (!(clip[j].w > 0.0f)) return true;
+        const f2 t = load_uv(mi, vi);
+        uvs[j] = {t.x * ts.x, t.y * ts.y};
+    }
+    const f2 win{float(W), float(H)};
+    const f2 ndc{(float(px) + 0.5f) / win.x * 2.0f - 1.0f, (float(py) + 0.5f) / win.y * 2.0f - 1.0f};
+    const BaryDeriv bary = calc_full_bary(clip[0], clip[1], clip[2], ndc, win);
+    Interp2 uv = interp2(bary, uvs[0], uvs[1], uvs[2]);
+    uv.v.x += mi.texture_offset.x;
+    uv.v.y += mi.texture_offset.y;
+    if (!(isfinite(uv.v.x) && isfinite(uv.v.y) && isfinite(uv.dx.x) && isfinite(uv.dx.y) && isfinite(uv.dy.x) &&
+          isfinite(uv.dy.y)))
+        return true;
+    return !(sample_texture(s, mi.base_color_texture_index, uv).w < 0.5f);
+}
+
 // Differentiable parameters: each pointer, when set, replaces the value the
 // scene or the uniforms hold (lh_resolve_params).
 LH_DEV f3 param_base_color(const lh_resolve_params& p, const MeshInfo& mi, uint32_t k) {

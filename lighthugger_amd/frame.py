@@ -24,6 +24,12 @@ def _align(n: int) -> int:
     return (n + ALIGN - 1) & ~(ALIGN - 1)
 
 
+def _scene_flags(mesh_infos: np.ndarray) -> int:
+    """LH_SCENE_OPAQUE_ONLY unless some mesh info is alpha-clipped."""
+    alpha = (np.asarray(mesh_infos["flags"]).astype(np.uint32) & abi.MESH_INFO_FLAGS_ALPHA_CLIP).any()
+    return 0 if alpha else abi.SCENE_OPAQUE_ONLY
+
+
 class HostMemory:
     kind = "host"
 
@@ -99,13 +105,15 @@ class SceneData:
         if len(tex):
             mem.write(self.handle, self.offsets["textures"], tex)
         self.c = abi.LhScene(o["instances"], o["mesh_infos"], o["factors"], o["meshlet_base"], o["meshlet_table"],
-                             o["textures"], n_in, n_mi, scene.total_meshlets, len(scene.textures))
+                             o["textures"], n_in, n_mi, scene.total_meshlets, len(scene.textures),
+                             _scene_flags(scene.mesh_infos))
 
     def set_factors(self, factors: np.ndarray) -> None:
         self.mem.write(self.handle, self.offsets["factors"], factors.astype(abi.MATERIAL_FACTORS))
 
     def set_mesh_infos(self, mesh_infos: np.ndarray) -> None:
         self.mem.write(self.handle, self.offsets["mesh_infos"], mesh_infos)
+        self.c.flags = _scene_flags(mesh_infos)
 
     def write_geometry(self, offset: int, data: np.ndarray) -> None:
         self.mem.write(self.handle, self.offsets["geometry"] + offset, data)

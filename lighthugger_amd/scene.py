@@ -38,6 +38,8 @@ def scene_lib() -> C.CDLL:
                                               C.c_uint32, abi.Vec2, abi.Vec2]
         lib.lh_scene_add_instance.restype = C.c_int32
         lib.lh_scene_add_instance.argtypes = [C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p]
+        lib.lh_scene_set_material_alpha_clip.restype = C.c_int32
+        lib.lh_scene_set_material_alpha_clip.argtypes = [C.c_void_p, C.c_uint32, C.c_uint32]
         lib.lh_scene_set_vertex_format.restype = C.c_int32
         lib.lh_scene_set_vertex_format.argtypes = [C.c_void_p, C.c_uint32]
         lib.lh_scene_finish.restype = C.c_int32
@@ -115,6 +117,7 @@ class Material:
     normal_texture: int = abi.UNUSED_TEXTURE_INDEX
     texture_scale: tuple = (1.0, 1.0)
     texture_offset: tuple = (0.0, 0.0)
+    alpha_clip: bool = False
 
 
 class HostScene:
@@ -188,6 +191,8 @@ class HostScene:
                                               abi.Vec2(*m.texture_scale), abi.Vec2(*m.texture_offset))
                 if r < 0:
                     raise ValueError("lh_scene_add_material failed")
+                if m.alpha_clip and lib.lh_scene_set_material_alpha_clip(h, r, 1) != 0:
+                    raise ValueError("lh_scene_set_material_alpha_clip failed")
             for mesh, mat, tr in instances:
                 tr = np.ascontiguousarray(tr, np.float32)
                 if lib.lh_scene_add_instance(h, mesh, mat, tr.ctypes.data) < 0:

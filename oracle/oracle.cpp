@@ -315,11 +315,22 @@ bool project(const CV& c, uint32_t W, uint32_t H, SV& o) {
     return true;
 }
 
+bool alpha_visible(const lh_scene& sc, const LhMat4& m, uint32_t W, uint32_t H, uint32_t id, int32_t px,
+                   int32_t py);
+
 struct RasterTarget {
     uint32_t W, H, kind;  // kind 0 main (u64 keys), 1 shadow (u32 depth)
     uint64_t* vis;
     uint32_t* depth;
     bool shared = false;  // several threads rasterize into the target: min by compare-exchange
+};
+
+// Alpha-clip geometry (MESH_INFO_FLAGS_ALPHA_CLIP): double-sided, alpha test
+// per covered sample (src/pipelines.cpp:22-25, rasterization.glsl:84-97,
+// :130-145).  NULL for opaque geometry.
+struct AlphaClip {
+    const lh_scene* sc;
+    const LhMat4* m;  // the pass's view-projection matrix
 };
 
 template <class T>
@@ -346,9 +357,9 @@ inline bool top_left(int32_t ax, int32_t ay, int32_t bx, int32_t by) {
 // < 0 in y-down framebuffer coordinates = counter-clockwise on screen,
 // src/pipelines.cpp:10-13 with the default VK_FRONT_FACE_COUNTER_CLOCKWISE);
 // shadows keep back faces (cull FRONT, src/pipelines.cpp:15-19).
-uint64_t raster_snapped(SV v0, SV v1, SV v2, const RasterTarget& t, uint32_t id) {
+uint64_t raster_snapped(SV v0, SV v1, SV v2, const RasterTarget& t, uint32_t id, const AlphaClip* ac) {
     int64_t area = edge(v0.X, v0.Y, v1.X, v1.Y, v2.X, v2.Y);
-    if (t.kind == 0 ? area >= 0 : area <= 0) return 0;
+    if (ac ? area == 0 : (t.kind == 0 ? area >= 0 : area <= 0)) return 0;
     if (area < 0) {
         std::swap(v1, v2);
         area = -area;
@@ -373,6 +384,7 @@ uint64_t raster_snapped(SV v0, SV v1, SV v2, const RasterTarget& t, uint32_t id)
             if (!(e0 > 0 || (e0 == 0 && tl0))) continue;
             if (!(e1 > 0 || (e1 == 0 && tl1))) continue;
             if (!(e2 > 0 || (e2 == 0 && tl2))) continue;
+            if (ac && !alpha_visible(*ac->sc, *ac->m, t.W, t.H, id, px, py)) continue;
             float l1 = float(e1) * inv_area, l2 = float(e2) * inv_area;
             float z = (v0.z + l1 * d10) + l2 * d20;
             z = std::min(std::max(z, 0.0f), 1.0f);
@@ -391,7 +403,7 @@ uint64_t raster_snapped(SV v0, SV v1, SV v2, const RasterTarget& t, uint32_t id)
     return frags;
 }
 
-uint64_t raster_clip_triangle(const CV* c, const RasterTarget& t, uint32_t id) {
+uint64_t raster_clip_triangle(const CV* c, const RasterTarget& t, uint32_t id, const AlphaClip* ac = nullptr) {
     if (!finite4(c[0]) || !finite4(c[1]) || !finite4(c[2])) return 0;
     uint32_t o0 = outcode(c[0]), o1 = outcode(c[1]), o2 = outcode(c[2]);
     if (o0 & o1 & o2) return 0;
@@ -399,7 +411,7 @@ uint64_t raster_clip_triangle(const CV* c, const RasterTarget& t, uint32_t id) {
         SV s[3];
         for (int i = 0; i < 3; i++)
             if (!project(c[i], t.W, t.H, s[i])) return 0;
-        return raster_snapped(s[0], s[1], s[2], t, id);
+        return raster_snapped(s[0], s[1], s[2], t, id, ac);
     }
     CV poly[12] = {c[0], c[1], c[2]};
     int n = clip_polygon(poly, 3);
@@ -408,7 +420,7 @@ uint64_t raster_clip_triangle(const CV* c, const RasterTarget& t, uint32_t id) {
     for (int i = 0; i < n; i++)
         if (!project(poly[i], t.W, t.H, s[i])) return 0;
     uint64_t frags = 0;
-    for (int i = 1; i + 1 < n; i++) frags += raster_snapped(s[0], s[i], s[i + 1], t, id);
+    for (int i = 1; i + 1 < n; i++) frags += raster_snapped(s[0], s[i], s[i + 1], t, id, ac);
     return frags;
 }
 
@@ -435,6 +447,8 @@ uint64_t raster_draws(const lh_scene& sc, const lh_view& v, uint32_t pass, uint3
         const MeshInfo& mi = mesh_info_of(in);
         const Meshlet& ml = meshlet_of(mi, ref.meshlet_index);
         uint32_t mid = sc.instance_meshlet_base[ref.instance_index] + ref.meshlet_index;
+        const AlphaClip alpha{&sc, m};
+        const AlphaClip* ac = (mi.flags & MESH_INFO_FLAGS_ALPHA_CLIP) ? &alpha : nullptr;
         for (uint32_t k = 0; k < ml.triangle_count; k++) {
             CV c[3];
             for (uint32_t j = 0; j < 3; j++) {
@@ -443,7 +457,7 @@ uint64_t raster_draws(const lh_scene& sc, const lh_view& v, uint32_t pass, uint3
                 F4 q = mat4_mul_vec4(*m, F4{w.x, w.y, w.z, 1.0f});
                 c[j] = CV{q.x, q.y, q.z, q.w};
             }
-            frags += raster_clip_triangle(c, t, (mid << LH_KEY_TRI_BITS) | k);
+            frags += raster_clip_triangle(c, t, (mid << LH_KEY_TRI_BITS) | k, ac);
         }
     }
     return frags;
@@ -590,6 +604,43 @@ F4 sample_texture(const lh_scene& sc, uint32_t index, const Interp2& uv) {
     F4 c0 = bilinear(t, lod.l0, uv.v);
     if (lod.f == 0.0f) return c0;
     return mix4(c0, bilinear(t, lod.l1, uv.v), lod.f);
+}
+
+// Alpha test of the alpha-clip pipelines (visbuffer_alpha_clip_pixel,
+// shadowmap_alpha_clipped_pixel, src/shaders/rasterization.glsl:84-97,
+// :130-145): base-colour alpha at the pixel centre, discarded below 0.5; the
+// uv is interpolated perspective-correctly with the LOD from its analytic
+// derivatives (CalcFullBary) in place of the hardware's 2x2-quad
+// differences.  A triangle with a vertex at or behind w = 0, or a non-finite
+// uv, passes.
+bool alpha_visible(const lh_scene& sc, const LhMat4& m, uint32_t W, uint32_t H, uint32_t id, int32_t px,
+                   int32_t py) {
+    const MeshletReference ref = sc.meshlet_table[id >> LH_KEY_TRI_BITS];
+    const Instance& in = sc.instances[ref.instance_index];
+    const MeshInfo& mi = mesh_info_of(in);
+    if (!has_texture(sc, mi.base_color_texture_index)) return true;
+    const Meshlet& ml = meshlet_of(mi, ref.meshlet_index);
+    const uint32_t tri = id & ((1u << LH_KEY_TRI_BITS) - 1);
+    F4 clip[3];
+    F2 uvs[3];
+    for (uint32_t j = 0; j < 3; j++) {
+        const uint32_t vi = corner_vertex(mi, ml, tri, j);
+        F3 w = mat4_point(in.transform, load_position(mi, vi));
+        clip[j] = mat4_mul_vec4(m, F4{w.x, w.y, w.z, 1.0f});
+        if (!(clip[j].w > 0.0f)) return true;
+        F2 t = load_uv(mi, vi);
+        uvs[j] = {t.x * mi.texture_scale.x, t.y * mi.texture_scale.y};
+    }
+    F2 win{float(W), float(H)};
+    F2 ndc{(float(px) + 0.5f) / win.x * 2.0f - 1.0f, (float(py) + 0.5f) / win.y * 2.0f - 1.0f};
+    BaryDeriv bary = calc_full_bary(clip[0], clip[1], clip[2], ndc, win);
+    Interp2 uv = interp2(bary, uvs[0], uvs[1], uvs[2]);
+    uv.v.x += mi.texture_offset.x;
+    uv.v.y += mi.texture_offset.y;
+    if (!(std::isfinite(uv.v.x) && std::isfinite(uv.v.y) && std::isfinite(uv.dx.x) && std::isfinite(uv.dx.y) &&
+          std::isfinite(uv.dy.x) && std::isfinite(uv.dy.y)))
+        return true;
+    return !(sample_texture(sc, mi.base_color_texture_index, uv).w < 0.5f);
 }
 
 // One tap of sampler2DArrayShadow with linear filtering, clamp to edge and

@@ -52,3 +52,45 @@ def fixtures():
     cam5, cfg5 = fixture_camera(128, 32, position=(0, 0, 1.6))
     out["strip_124t"] = (HostScene.build([_strip_124()], [mat], [(0, 0, np.eye(4, dtype=np.float32))]), cam5, cfg5)
     return out
+
+
+def alpha_texture(n=32, seed=1, cell=4):
+    """RGBA8 texture whose alpha is 0 or 255 in cell x cell blocks (noise RGB)."""
+    rng = np.random.default_rng(seed)
+    cells = rng.random((n // cell, n // cell)) < 0.5
+    a = (np.kron(cells, np.ones((cell, cell))) * 255).astype(np.uint8)
+    rgb = rng.integers(64, 256, (n, n, 3)).astype(np.uint8)
+    return np.dstack([rgb, a])
+
+
+def _card(scale, yaw, pitch, t):
+    """Quad transform: scale, then pitch about x, yaw about y, then translate."""
+    cy, sy, cp, sp = np.cos(yaw), np.sin(yaw), np.cos(pitch), np.sin(pitch)
+    ry = np.array([[cy, 0, sy], [0, 1, 0], [-sy, 0, cy]])
+    rx = np.array([[1, 0, 0], [0, cp, -sp], [0, sp, cp]])
+    m = np.eye(4)
+    m[:3, :3] = ry @ rx * scale
+    m[:3, 3] = t
+    return m.T.astype(np.float32)  # column-major
+
+
+def alpha_scene(width=96, height=64, shadow=128):
+    """Alpha-clipped cards (double-sided, base-colour alpha < 0.5 discarded)
+    over an opaque ground: cards facing towards and away from the camera, one
+    close enough for the large-triangle path, one through the near plane, one
+    alpha-clipped card without a texture (alpha 1), all casting shadows."""
+    ground = Material(base_color=(0.6, 0.6, 0.6), roughness=0.8, metallic=0.0)
+    card = Material(base_color=(0.9, 0.8, 0.7), roughness=0.5, metallic=0.0, base_color_texture=0, alpha_clip=True)
+    plain = Material(base_color=(0.3, 0.7, 0.4), roughness=0.4, metallic=0.0, alpha_clip=True)
+    rng = np.random.default_rng(7)
+    inst = [(0, 0, _card(6.0, 0.0, -np.pi / 2, (0, -1.0, 0)))]
+    for _ in range(10):
+        t = (rng.uniform(-2.5, 2.5), rng.uniform(-0.6, 1.2), rng.uniform(-2.5, 1.0))
+        inst.append((1, 1, _card(rng.uniform(0.4, 1.0), rng.uniform(0, 2 * np.pi), rng.uniform(-0.5, 0.5), t)))
+    inst.append((1, 1, _card(1.2, 0.3, 0.0, (0.4, 0.3, 2.6))))           # near the camera: large triangles
+    inst.append((1, 1, _card(1.5, np.pi / 2, 0.0, (0.3, 0.0, 3.5))))     # along the view axis, through the near plane
+    inst.append((1, 2, _card(0.5, 2.5, 0.2, (-1.2, 0.2, 0.5))))          # no texture: alpha 1
+    scene = HostScene.build([quad_mesh(1.0), grid_patch(6, 6)], [ground, card, plain], inst,
+                            textures=[alpha_texture()])
+    cam, cfg = fixture_camera(width, height, position=(0.2, 0.4, 4.0), target=(0, 0, 0), near=0.1, shadow=shadow)
+    return scene, cam, cfg

@@ -31,7 +31,7 @@ int main(void) {
   S(LhTexture) S(LhMaterialFactors) S(lh_scene) S(lh_view) S(lh_camera) S(lh_scene_arrays) S(lh_tuning)
   O(MeshInfo, base_color_factor) O(Instance, normal_transform) O(Meshlet, index_count)
   O(Uniforms, debug_shadowmaps) O(MiscStorage, max_depth) O(LhTexture, level_offset)
-  O(lh_scene, num_instances) O(lh_scene, num_textures) O(lh_view, scratch_bytes) O(lh_view, instance_capacity)
+  O(lh_scene, num_instances) O(lh_scene, num_textures) O(lh_scene, flags) O(lh_view, scratch_bytes) O(lh_view, instance_capacity)
   O(lh_view, flags) O(lh_view, history) O(lh_camera, shadow_cam_distance) O(lh_scene_arrays, unique_triangles)
   return 0;
 }
@@ -74,6 +74,7 @@ def test_field_offsets(probe):
     assert probe["LhTexture.level_offset"] == abi.TEXTURE.fields["level_offset"][1]
     assert probe["lh_scene.num_instances"] == abi.LhScene.num_instances.offset
     assert probe["lh_scene.num_textures"] == abi.LhScene.num_textures.offset
+    assert probe["lh_scene.flags"] == abi.LhScene.flags.offset
     assert probe["lh_view.scratch_bytes"] == abi.LhView.scratch_bytes.offset
     assert probe["lh_view.instance_capacity"] == abi.LhView.instance_capacity.offset
     assert probe["lh_view.flags"] == abi.LhView.flags.offset
