@@ -13,6 +13,9 @@ MESH_INFO_FLAGS_32_BIT_INDICES = 0x01
 MESH_INFO_FLAGS_ALPHA_CLIP = 0x02
 MESH_INFO_FLAGS_FP32_VERTICES = 0x04
 SCENE_OPAQUE_ONLY = 0x1  # LH_SCENE_OPAQUE_ONLY
+# Uniforms.debug (src/shared_cpu_gpu.h:46-51)
+(UNIFORMS_DEBUG_OFF, UNIFORMS_DEBUG_CASCADES, UNIFORMS_DEBUG_TRIANGLE_INDEX, UNIFORMS_DEBUG_INSTANCE_INDEX,
+ UNIFORMS_DEBUG_SHADER_CLOCK, UNIFORMS_DEBUG_NORMALS) = range(6)
 UNUSED_TEXTURE_INDEX = 0xFFFF
 MAX_MESHLET_UNIQUE_VERTICES = 64
 MAX_MESHLET_TRIANGLES = 124

@@ -31,7 +31,7 @@ LH_DEV void resolve_tile(const lh_scene& s, const lh_view& v, uint32_t tx, uint3
     const f3 c = shading::resolve_pixel(s, v, x, y, p);
     const size_t o = size_t(y) * v.width + x;
     reinterpret_cast<float4*>(v.radiance)[o] = make_float4(c.x, c.y, c.z, 1.0f);
-    if (v.ldr) v.ldr[o] = shading::tonemap_pixel(c);
+    if (v.ldr) v.ldr[o] = shading::display_pixel(v, x, y, c);
 }
 
 }  // namespace
@@ -62,7 +62,7 @@ __global__ void __launch_bounds__(256) k_tonemap(VS vs) {
     const uint64_t n = uint64_t(v.width) * v.height;
     for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += uint64_t(gridDim.x) * blockDim.x) {
         float4 c = reinterpret_cast<const float4*>(v.radiance)[i];
-        v.ldr[i] = shading::tonemap_pixel(f3{c.x, c.y, c.z});
+        v.ldr[i] = shading::display_pixel(v, uint32_t(i % v.width), uint32_t(i / v.width), f3{c.x, c.y, c.z});
     }
 }
 

@@ -325,7 +325,7 @@ LH_DEV f3 param_sun_intensity(const lh_resolve_params& p, const Uniforms& u) {
 }
 
 // Cascade selection and 3x3 PCF (render_geometry.comp:183-226).
-LH_DEV float shadow_factor(const lh_view& v, const MeshInfo& mi, f3 world) {
+LH_DEV float shadow_factor(const lh_view& v, const MeshInfo& mi, f3 world, uint32_t* cascade_out = nullptr) {
     const MiscStorage& misc = *v.misc;
     uint32_t cascade;
     f4 sc{0, 0, 0, 0};
@@ -333,6 +333,7 @@ LH_DEV float shadow_factor(const lh_view& v, const MeshInfo& mi, f3 world) {
         sc = mat4_mul_vec4(misc.uv_space_shadow_matrices[cascade], f4{world.x, world.y, world.z, 1.0f});
         if (fabsf(sc.x - 0.5f) < 0.5f && fabsf(sc.y - 0.5f) < 0.5f) break;
     }
+    if (cascade_out) *cascade_out = cascade;  // 4: outside every cascade
     const float bias = (mi.flags & MESH_INFO_FLAGS_ALPHA_CLIP) ? 0.000002f : 0.0f;
     sc = f4{sc.x / sc.w, sc.y / sc.w, sc.z / sc.w, sc.w / sc.w};
     const uint32_t layer = min(cascade, 3u);
@@ -356,12 +357,29 @@ LH_DEV float ambient_sh(f3 nrm) {
            0.0f * (nrm.x * nrm.x - nrm.y * nrm.y);
 }
 
+// Debug colours (src/shaders/common/debug.glsl:1-11).
+LH_DEV f3 debug_colour(uint32_t i) {
+    switch (i) {
+        case 0: return {0.0f / 255.0f, 2.0f / 255.0f, 91.0f / 255.0f};
+        case 1: return {0.0f / 255.0f, 108.0f / 255.0f, 251.0f / 255.0f};
+        case 2: return {0.0f / 255.0f, 221.0f / 255.0f, 221.0f / 255.0f};
+        case 3: return {51.0f / 255.0f, 221.0f / 255.0f, 0.0f / 255.0f};
+        case 4: return {255.0f / 255.0f, 252.0f / 255.0f, 0.0f / 255.0f};
+        case 5: return {255.0f / 255.0f, 180.0f / 255.0f, 0.0f / 255.0f};
+        case 6: return {255.0f / 255.0f, 104.0f / 255.0f, 0.0f / 255.0f};
+        case 7: return {226.0f / 255.0f, 22.0f / 255.0f, 0.0f / 255.0f};
+        case 8: return {191.0f / 255.0f, 0.0f / 255.0f, 83.0f / 255.0f};
+        default: return {145.0f / 255.0f, 0.0f / 255.0f, 65.0f / 255.0f};
+    }
+}
+
 LH_DEV f3 resolve_pixel(const lh_scene& s, const lh_view& v, uint32_t x, uint32_t y, const lh_resolve_params& prm) {
     const Uniforms& u = *v.uniforms;
     const f2 win{float(u.window_size.x), float(u.window_size.y)};
     const f2 ndc{float(x) / win.x * 2.0f - 1.0f, float(y) / win.y * 2.0f - 1.0f};  // pixel corner
     Surface sf;
-    if (!decode(s, v, v.visbuffer[size_t(y) * v.width + x], sf)) return {kSkyR, kSkyG, kSkyB};
+    const uint64_t key = v.visbuffer[size_t(y) * v.width + x];
+    if (!decode(s, v, key, sf)) return {kSkyR, kSkyG, kSkyB};
     const Instance& in = s.instances[sf.instance];
     const MeshInfo& mi = s.mesh_infos[sf.mesh_info];
     const BaryDeriv bary = calc_full_bary(sf.clip[0], sf.clip[1], sf.clip[2], ndc, win);
@@ -383,7 +401,8 @@ LH_DEV f3 resolve_pixel(const lh_scene& s, const lh_view& v, uint32_t x, uint32_
     const f3 gn = cross3(sub(sf.pos[1], sf.pos[0]), sub(sf.pos[2], sf.pos[0]));
     const bool flip = dot3(gn, view_vector) < 0.0f;
     if (flip) nrm = neg(nrm);
-    const float shadow_sum = shadow_factor(v, mi, world.v);
+    uint32_t cascade = 0;
+    const float shadow_sum = shadow_factor(v, mi, world.v, &cascade);
     const Lod lod_b = lod_of(s, mi.base_color_texture_index, uv), lod_m = lod_of(s, mi.metallic_roughness_texture_index, uv);
     if (prm.frozen)
         reinterpret_cast<uint4*>(prm.frozen)[size_t(y) * v.width + x] = pack_frozen(shadow_sum, flip, lod_b, lod_m);
@@ -405,6 +424,18 @@ LH_DEV f3 resolve_pixel(const lh_scene& s, const lh_view& v, uint32_t x, uint32_
         nrm = normalize3(f3{(t.x * map.x + b.x * map.y) + nrm.x * map.z, (t.y * map.x + b.y * map.y) + nrm.y * map.z,
                             (t.z * map.x + b.z * map.y) + nrm.z * map.z});
     }
+    // Debug views (render_geometry.comp:268-308); the shader-clock heat map
+    // (UNIFORMS_DEBUG_SHADER_CLOCK) is not reproduced and shades normally.
+    switch (u.debug) {
+        case UNIFORMS_DEBUG_CASCADES: return mul(base, debug_colour(cascade));
+        case UNIFORMS_DEBUG_TRIANGLE_INDEX: return debug_colour(uint32_t(key & ((1u << LH_KEY_TRI_BITS) - 1)) % 10u);
+        case UNIFORMS_DEBUG_INSTANCE_INDEX:
+            return debug_colour(
+                uint32_t(s.meshlet_table[uint32_t(key >> LH_KEY_TRI_BITS) & (LH_MAX_SCENE_MESHLETS - 1)].meshlet_index) %
+                10u);
+        case UNIFORMS_DEBUG_NORMALS: return {nrm.x * 0.5f + 0.5f, nrm.y * 0.5f + 0.5f, nrm.z * 0.5f + 0.5f};
+        default: break;
+    }
     const f3 ambient = mul(mul(base, 1.0f - metallic), ambient_sh(nrm));
     const f3 sun = mul(mul(brdf(view_vector, param_sun_dir(prm, u), nrm, roughness, metallic, base,
                                 (mi.flags & MESH_INFO_FLAGS_ALPHA_CLIP) != 0),
@@ -420,5 +451,41 @@ LH_DEV uint32_t encode8(float x) {
     return uint32_t(t * 255.0f + 0.5f);
 }
 LH_DEV uint32_t tonemap_pixel(f3 c) { return encode8(c.z) | (encode8(c.y) << 8) | (encode8(c.x) << 16) | (255u << 24); }
+
+// Shadow-map depth with linear filtering and clamp to edge (textureLod of
+// the depth array at level 0).
+LH_DEV float shadow_depth(const lh_view& v, uint32_t layer, f2 uv) {
+    const int32_t S = int32_t(v.shadow_size);
+    const uint32_t* map = v.shadow_maps + size_t(layer) * v.shadow_size * v.shadow_size;
+    const float fx = uv.x * float(S) - 0.5f, fy = uv.y * float(S) - 0.5f;
+    const float x0 = floorf(fx), y0 = floorf(fy);
+    const float a = fx - x0, b = fy - y0;
+    const int32_t i = int32_t(x0), j = int32_t(y0);
+    auto at = [&](int32_t ii, int32_t jj) {
+        ii = min(max(ii, 0), S - 1);
+        jj = min(max(jj, 0), S - 1);
+        return __uint_as_float(map[size_t(jj) * v.shadow_size + uint32_t(ii)]);
+    };
+    return mixf(mixf(at(i, j), at(i + 1, j), a), mixf(at(i, j + 1), at(i + 1, j + 1), a), b);
+}
+
+// Display transform of one pixel (display_transform.comp:37-81): the tone
+// curve and sRGB encoding, or, with Uniforms.debug_shadowmaps, the four
+// cascades in the top-left quarter of the window (2 x 2, depth d shown as
+// (d - 0.5) * 100 + 0.5 without the tone curve).
+LH_DEV uint32_t display_pixel(const lh_view& v, uint32_t x, uint32_t y, f3 c) {
+    const Uniforms& u = *v.uniforms;
+    if (u.debug_shadowmaps) {
+        const float ux = (float(x) + 0.5f) / float(u.window_size.x), uy = (float(y) + 0.5f) / float(u.window_size.y);
+        if (ux < 0.5f && uy < 0.5f) {
+            const float sx = ux * 2.0f / 0.5f, sy = uy * 2.0f / 0.5f;
+            const uint32_t layer = (sy > 1.0f ? 2u : 0u) + (sx > 1.0f ? 1u : 0u);
+            const float g = (shadow_depth(v, layer, f2{sx - floorf(sx), sy - floorf(sy)}) - 0.5f) * 100.0f + 0.5f;
+            const uint32_t e = uint32_t(clampf(srgb_oetf(fmaxf(g, 0.0f)), 0.0f, 1.0f) * 255.0f + 0.5f);
+            return e | (e << 8) | (e << 16) | (255u << 24);
+        }
+    }
+    return tonemap_pixel(c);
+}
 
 }  // namespace shading

@@ -721,7 +721,7 @@ const F3 kSky{0.35f * 8.0f, 0.50f * 8.0f, 0.80f * 8.0f};
 
 // Cascade selection (render_geometry.comp:183-196; the loop may end at 4,
 // which the array layer clamps to 3) and 3x3 PCF (:206-226).
-float shadow_factor(const lh_view& v, const MeshInfo& mi, F3 world) {
+float shadow_factor(const lh_view& v, const MeshInfo& mi, F3 world, uint32_t* cascade_out = nullptr) {
     const MiscStorage& misc = *v.misc;
     uint32_t cascade;
     F4 sc4{0, 0, 0, 0};
@@ -730,6 +730,7 @@ float shadow_factor(const lh_view& v, const MeshInfo& mi, F3 world) {
         float ox = sc4.x - 0.5f, oy = sc4.y - 0.5f;
         if (std::fabs(ox) < 0.5f && std::fabs(oy) < 0.5f) break;
     }
+    if (cascade_out) *cascade_out = cascade;
     float bias = (mi.flags & MESH_INFO_FLAGS_ALPHA_CLIP) ? 0.000002f : 0.0f;
     sc4 = F4{sc4.x / sc4.w, sc4.y / sc4.w, sc4.z / sc4.w, sc4.w / sc4.w};
     uint32_t layer = std::min(cascade, 3u);
@@ -743,6 +744,14 @@ float shadow_factor(const lh_view& v, const MeshInfo& mi, F3 world) {
     shadow_sum /= 9.0f;
     if (sc4.z > 1.0f) shadow_sum = 1.0f;
     return shadow_sum;
+}
+
+// DEBUG_COLOURS (src/shaders/common/debug.glsl:1-11).
+F3 debug_colour(uint32_t i) {
+    static const float k[10][3] = {{0, 2, 91},    {0, 108, 251}, {0, 221, 221}, {51, 221, 0},  {255, 252, 0},
+                                   {255, 180, 0}, {255, 104, 0}, {226, 22, 0},  {191, 0, 83}, {145, 0, 65}};
+    const float* c = k[std::min(i, 9u)];
+    return {c[0] / 255.0f, c[1] / 255.0f, c[2] / 255.0f};
 }
 
 // render_geometry main() (src/shaders/render_geometry.comp:70-266) for one pixel.
@@ -785,7 +794,8 @@ F3 resolve_pixel(const lh_scene& sc, const lh_view& v, uint32_t x, uint32_t y) {
     nrm = normalize3(mat3_mul_vec3(in.normal_transform, nrm));
     F3 gn = cross3(sub(pos[1], pos[0]), sub(pos[2], pos[0]));
     if (dot3(gn, view_vector) < 0.0f) nrm = neg(nrm);
-    float shadow_sum = shadow_factor(v, mi, world.v);
+    uint32_t cascade = 0;
+    float shadow_sum = shadow_factor(v, mi, world.v, &cascade);
     // Material (:228-246).
     const LhMaterialFactors& fac = sc.factors[(in.mesh_info_address - uint64_t(uintptr_t(sc.mesh_infos))) / sizeof(MeshInfo)];
     F4 bc = sample_texture(sc, mi.base_color_texture_index, uv);
@@ -807,6 +817,11 @@ F3 resolve_pixel(const lh_scene& sc, const lh_view& v, uint32_t x, uint32_t y) {
         nrm = normalize3(F3{(t.x * map.x + b.x * map.y) + nrm.x * map.z, (t.y * map.x + b.y * map.y) + nrm.y * map.z,
                             (t.z * map.x + b.z * map.y) + nrm.z * map.z});
     }
+    // Debug views (:268-308); the shader-clock heat map is not reproduced.
+    if (u.debug == UNIFORMS_DEBUG_CASCADES) return mul(base, debug_colour(cascade));
+    if (u.debug == UNIFORMS_DEBUG_TRIANGLE_INDEX) return debug_colour(tri % 10u);
+    if (u.debug == UNIFORMS_DEBUG_INSTANCE_INDEX) return debug_colour(uint32_t(ref.meshlet_index) % 10u);
+    if (u.debug == UNIFORMS_DEBUG_NORMALS) return {nrm.x * 0.5f + 0.5f, nrm.y * 0.5f + 0.5f, nrm.z * 0.5f + 0.5f};
     // Ambient SH (:40-57, :248-249).
     const float L = 0.025f;
     float shv = (((((((L + 0.0f * nrm.x) + L * nrm.y) + 0.0f * nrm.z) + 0.0f * (nrm.x * nrm.z)) + 0.0f * (nrm.z * nrm.y)) +
@@ -834,6 +849,40 @@ uint32_t tonemap_pixel(F3 c) {
         return uint32_t(t * 255.0f + 0.5f);
     };
     return enc(c.z) | (enc(c.y) << 8) | (enc(c.x) << 16) | (255u << 24);
+}
+
+// textureLod of the shadow-map array at level 0: linear filter, clamp to edge.
+float shadow_depth(const lh_view& v, uint32_t layer, F2 uv) {
+    const int32_t S = int32_t(v.shadow_size);
+    const uint32_t* map = v.shadow_maps + size_t(layer) * v.shadow_size * v.shadow_size;
+    float fx = uv.x * float(S) - 0.5f, fy = uv.y * float(S) - 0.5f;
+    float x0 = std::floor(fx), y0 = std::floor(fy);
+    float a = fx - x0, b = fy - y0;
+    int32_t i = int32_t(x0), j = int32_t(y0);
+    auto at = [&](int32_t ii, int32_t jj) {
+        ii = std::min(std::max(ii, 0), S - 1);
+        jj = std::min(std::max(jj, 0), S - 1);
+        return bitsf(map[size_t(jj) * v.shadow_size + uint32_t(ii)]);
+    };
+    return mixf(mixf(at(i, j), at(i + 1, j), a), mixf(at(i, j + 1), at(i + 1, j + 1), a), b);
+}
+
+// display_transform (:37-81) for pixel (x, y): with Uniforms.debug_shadowmaps
+// the top-left quarter shows the four cascades (2 x 2), depth d as
+// (d - 0.5) * 100 + 0.5 without the tone curve (:51-75).
+uint32_t display_pixel(const lh_view& v, uint32_t x, uint32_t y, F3 c) {
+    const Uniforms& u = *v.uniforms;
+    if (u.debug_shadowmaps) {
+        float ux = (float(x) + 0.5f) / float(u.window_size.x), uy = (float(y) + 0.5f) / float(u.window_size.y);
+        if (ux < 0.5f && uy < 0.5f) {
+            float sx = ux * 2.0f / 0.5f, sy = uy * 2.0f / 0.5f;
+            uint32_t layer = (sy > 1.0f ? 2u : 0u) + (sx > 1.0f ? 1u : 0u);
+            float g = (shadow_depth(v, layer, F2{sx - std::floor(sx), sy - std::floor(sy)}) - 0.5f) * 100.0f + 0.5f;
+            uint32_t e = uint32_t(clampf(srgb_oetf(std::max(g, 0.0f)), 0.0f, 1.0f) * 255.0f + 0.5f);
+            return e | (e << 8) | (e << 16) | (255u << 24);
+        }
+    }
+    return tonemap_pixel(c);
 }
 
 bool valid(const lh_scene* sc, const lh_view* v) {
@@ -1149,7 +1198,7 @@ static void resolve_rows(const lh_scene* sc, const lh_view* v, uint32_t y0, uint
             o[1] = c.y;
             o[2] = c.z;
             o[3] = 1.0f;
-            if (v->ldr) v->ldr[size_t(y) * v->width + x] = tonemap_pixel(c);
+            if (v->ldr) v->ldr[size_t(y) * v->width + x] = display_pixel(*v, x, y, c);
         }
 }
 
@@ -1163,7 +1212,7 @@ lh_status orc_tonemap(const lh_view* v) {
     if (!v || !v->radiance || !v->ldr) return LH_ERR_INVALID;
     for (size_t p = 0; p < size_t(v->width) * v->height; p++) {
         const float* c = v->radiance + 4 * p;
-        v->ldr[p] = tonemap_pixel(F3{c[0], c[1], c[2]});
+        v->ldr[p] = display_pixel(*v, uint32_t(p % v->width), uint32_t(p / v->width), F3{c[0], c[1], c[2]});
     }
     return LH_OK;
 }

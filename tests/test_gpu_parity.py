@@ -87,6 +87,30 @@ def test_alpha_clip_frame_parity():
     assert gpu["counters"][abi.CNT_BIG] > 0 and gpu["counters"][abi.CNT_CLIP] > 0
 
 
+@pytest.mark.parametrize("mode", [abi.UNIFORMS_DEBUG_CASCADES, abi.UNIFORMS_DEBUG_TRIANGLE_INDEX,
+                                  abi.UNIFORMS_DEBUG_INSTANCE_INDEX, abi.UNIFORMS_DEBUG_NORMALS])
+def test_debug_views_match_oracle(mode):
+    """Uniforms.debug views of the resolve and (with the normals view) the
+    shadow-map overlay of the display transform, against the oracle."""
+    import torch
+
+    from lighthugger_amd.lib import Renderer
+
+    scene = HostScene.preset(abi.PRESET_SMALL)
+    cam = preset_camera(abi.PRESET_SMALL)
+    cfg = ViewConfig(cam.width, cam.height, 512)
+    u = make_uniforms(cam, scene.num_instances)
+    u["debug"] = mode
+    u["debug_shadowmaps"] = 1 if mode == abi.UNIFORMS_DEBUG_NORMALS else 0
+    r = Renderer(scene, cfg)
+    r.view.set_uniforms(u, r.scene)
+    r.frame()
+    torch.cuda.synchronize()
+    gpu = snapshot(r.view)
+    orc = oracle_frame(scene, cfg, u)
+    print(mode, compare_frames(gpu, orc, scene), flush=True)
+
+
 def test_fixtures_match_oracle_and_golden():
     import hashlib
     import json
