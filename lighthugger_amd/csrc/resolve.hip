@@ -40,8 +40,9 @@ __global__ void __launch_bounds__(64) LH_RESOLVE_OCCUPANCY k_resolve(lh_scene s,
     resolve_tile(s, vs.v, blockIdx.x, blockIdx.y, p);
 }
 
-__global__ void __launch_bounds__(64) LH_RESOLVE_OCCUPANCY k_resolve_batch(lh_scene s, ViewTable vs,
-                                                                            lh_resolve_params p) {
+// Four waves per SIMD: without the target the batch variant's view lookup
+// takes it one register past 128 (three waves).
+__global__ void __launch_bounds__(64) LH_WAVES_ATTR(4) k_resolve_batch(lh_scene s, ViewTable vs, lh_resolve_params p) {
     const uint32_t g = blockIdx.x;
     uint32_t lo = 0, hi = vs.n;  // last view with tile_offsets[view] <= g
     while (hi - lo > 1) {

@@ -23,10 +23,10 @@ from pathlib import Path
 
 # Kernel symbols that are unique to one entry of lh_frame_profiled's names.
 FRAME_KERNELS = {
-    "raster_meshlets_view": "k_raster_meshlets<0, lhd::OneView, false>",
-    "raster_meshlets_shadow": "k_raster_meshlets<1, lhd::OneView, false>",
-    "raster_big_view": "k_raster_big<0, lhd::OneView, false>",
-    "raster_big_shadow": "k_raster_big<1, lhd::OneView, false>",
+    "raster_meshlets_view": "k_raster_meshlets<0, lhd::OneView, false, false>",
+    "raster_meshlets_shadow": "k_raster_meshlets<1, lhd::OneView, false, false>",
+    "raster_big_view": "k_raster_big<0, lhd::OneView, false, false>",
+    "raster_big_shadow": "k_raster_big<1, lhd::OneView, false, false>",
     "raster_clip_view": "k_raster_clip<0, lhd::OneView>",
     "raster_clip_shadow": "k_raster_clip<1, lhd::OneView>",
     "resolve": "k_resolve",
