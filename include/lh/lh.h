@@ -124,7 +124,11 @@ typedef struct lh_view {
     uint32_t* history;          /* LH_VIEW_HIZ: one bit per stable meshlet id (total_meshlets bits),
                                    set for the meshlets that own a pixel of the last frame of this
                                    view; may be NULL.  Zero it when the view is created. */
+    const float* tonemap_lut;   /* optional display LUT (tony_mc_mapface, display_transform.comp:5-17):
+                                   LH_TONEMAP_LUT_DIMS^3 RGBA float entries, red fastest, indexed by the
+                                   stimulus encoded as x / (x + 1); NULL: that encoding is the curve */
 } lh_view;
+#define LH_TONEMAP_LUT_DIMS 48u
 
 /* Bytes of scratch needed for the given queue capacities. */
 uint64_t lh_scratch_bytes(uint32_t big_capacity, uint32_t clip_capacity);

@@ -13,6 +13,7 @@ MESH_INFO_FLAGS_32_BIT_INDICES = 0x01
 MESH_INFO_FLAGS_ALPHA_CLIP = 0x02
 MESH_INFO_FLAGS_FP32_VERTICES = 0x04
 SCENE_OPAQUE_ONLY = 0x1  # LH_SCENE_OPAQUE_ONLY
+TONEMAP_LUT_DIMS = 48  # LH_TONEMAP_LUT_DIMS
 # Uniforms.debug (src/shared_cpu_gpu.h:46-51)
 (UNIFORMS_DEBUG_OFF, UNIFORMS_DEBUG_CASCADES, UNIFORMS_DEBUG_TRIANGLE_INDEX, UNIFORMS_DEBUG_INSTANCE_INDEX,
  UNIFORMS_DEBUG_SHADER_CLOCK, UNIFORMS_DEBUG_NORMALS) = range(6)
@@ -143,7 +144,7 @@ class LhView(C.Structure):
         ("radiance", C.c_uint64), ("ldr", C.c_uint64), ("scratch", C.c_uint64), ("scratch_bytes", C.c_uint64),
         ("instance_capacity", C.c_uint32), ("draw_capacity", C.c_uint32), ("width", C.c_uint32),
         ("height", C.c_uint32), ("shadow_size", C.c_uint32), ("big_capacity", C.c_uint32),
-        ("clip_capacity", C.c_uint32), ("flags", C.c_uint32), ("history", C.c_uint64),
+        ("clip_capacity", C.c_uint32), ("flags", C.c_uint32), ("history", C.c_uint64), ("tonemap_lut", C.c_uint64),
     ]
 
 

@@ -469,10 +469,44 @@ LH_DEV float shadow_depth(const lh_view& v, uint32_t layer, f2 uv) {
     return mixf(mixf(at(i, j), at(i + 1, j), a), mixf(at(i, j + 1), at(i + 1, j + 1), a), b);
 }
 
+// tony_mc_mapface (display_transform.comp:5-17): the stimulus encoded as
+// x / (x + 1), aligned to texel centres, looked up with trilinear filtering
+// and clamp to edge in the view's LH_TONEMAP_LUT_DIMS^3 RGBA float LUT.
+LH_DEV f3 lut_lookup(const float* lut, f3 c) {
+    constexpr int32_t N = int32_t(LH_TONEMAP_LUT_DIMS);
+    constexpr float dims = float(LH_TONEMAP_LUT_DIMS);
+    const float x[3] = {fmaxf(c.x, 0.0f), fmaxf(c.y, 0.0f), fmaxf(c.z, 0.0f)};
+    int32_t i0[3];
+    float fr[3];
+    for (int k = 0; k < 3; k++) {
+        const float uvw = (x[k] / (x[k] + 1.0f)) * ((dims - 1.0f) / dims) + 0.5f / dims;
+        const float t = uvw * dims - 0.5f, fl = floorf(t);
+        fr[k] = t - fl;
+        i0[k] = int32_t(fl);
+    }
+    auto at = [&](int32_t r, int32_t g, int32_t b) {
+        r = min(max(r, 0), N - 1);
+        g = min(max(g, 0), N - 1);
+        b = min(max(b, 0), N - 1);
+        const float4 q = reinterpret_cast<const float4*>(lut)[(size_t(b) * N + g) * N + r];
+        return f3{q.x, q.y, q.z};
+    };
+    auto mix3 = [](f3 a, f3 b, float t) { return f3{mixf(a.x, b.x, t), mixf(a.y, b.y, t), mixf(a.z, b.z, t)}; };
+    f3 plane[2];
+    for (int db = 0; db < 2; db++) {
+        const f3 g0 = mix3(at(i0[0], i0[1], i0[2] + db), at(i0[0] + 1, i0[1], i0[2] + db), fr[0]);
+        const f3 g1 = mix3(at(i0[0], i0[1] + 1, i0[2] + db), at(i0[0] + 1, i0[1] + 1, i0[2] + db), fr[0]);
+        plane[db] = mix3(g0, g1, fr[1]);
+    }
+    return mix3(plane[0], plane[1], fr[2]);
+}
+LH_DEV uint32_t encode_display(float x) { return uint32_t(clampf(srgb_oetf(x), 0.0f, 1.0f) * 255.0f + 0.5f); }
+
 // Display transform of one pixel (display_transform.comp:37-81): the tone
-// curve and sRGB encoding, or, with Uniforms.debug_shadowmaps, the four
-// cascades in the top-left quarter of the window (2 x 2, depth d shown as
-// (d - 0.5) * 100 + 0.5 without the tone curve).
+// curve (the view's LUT, or x / (x + 1) without one) and sRGB encoding, or,
+// with Uniforms.debug_shadowmaps, the four cascades in the top-left quarter
+// of the window (2 x 2, depth d shown as (d - 0.5) * 100 + 0.5 without the
+// tone curve).
 LH_DEV uint32_t display_pixel(const lh_view& v, uint32_t x, uint32_t y, f3 c) {
     const Uniforms& u = *v.uniforms;
     if (u.debug_shadowmaps) {
@@ -484,6 +518,10 @@ LH_DEV uint32_t display_pixel(const lh_view& v, uint32_t x, uint32_t y, f3 c) {
             const uint32_t e = uint32_t(clampf(srgb_oetf(fmaxf(g, 0.0f)), 0.0f, 1.0f) * 255.0f + 0.5f);
             return e | (e << 8) | (e << 16) | (255u << 24);
         }
+    }
+    if (v.tonemap_lut) {
+        const f3 d = lut_lookup(v.tonemap_lut, c);
+        return encode_display(d.z) | (encode_display(d.y) << 8) | (encode_display(d.x) << 16) | (255u << 24);
     }
     return tonemap_pixel(c);
 }

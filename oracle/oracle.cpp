@@ -867,6 +867,38 @@ float shadow_depth(const lh_view& v, uint32_t layer, F2 uv) {
     return mixf(mixf(at(i, j), at(i + 1, j), a), mixf(at(i, j + 1), at(i + 1, j + 1), a), b);
 }
 
+// tony_mc_mapface (display_transform.comp:5-17) with the view's LUT:
+// x / (x + 1) aligned to texel centres, trilinear, clamp to edge.
+F3 lut_lookup(const float* lut, F3 c) {
+    const int32_t N = int32_t(LH_TONEMAP_LUT_DIMS);
+    const float dims = float(LH_TONEMAP_LUT_DIMS);
+    const float x[3] = {std::max(c.x, 0.0f), std::max(c.y, 0.0f), std::max(c.z, 0.0f)};
+    int32_t i0[3];
+    float fr[3];
+    for (int k = 0; k < 3; k++) {
+        float uvw = (x[k] / (x[k] + 1.0f)) * ((dims - 1.0f) / dims) + 0.5f / dims;
+        float t = uvw * dims - 0.5f, fl = std::floor(t);
+        fr[k] = t - fl;
+        i0[k] = int32_t(fl);
+    }
+    auto at = [&](int32_t r, int32_t g, int32_t b) {
+        r = std::min(std::max(r, 0), N - 1);
+        g = std::min(std::max(g, 0), N - 1);
+        b = std::min(std::max(b, 0), N - 1);
+        const float* q = lut + 4 * ((size_t(b) * N + g) * N + r);
+        return F3{q[0], q[1], q[2]};
+    };
+    auto mix3 = [](F3 a, F3 b, float t) { return F3{mixf(a.x, b.x, t), mixf(a.y, b.y, t), mixf(a.z, b.z, t)}; };
+    F3 plane[2];
+    for (int db = 0; db < 2; db++) {
+        F3 g0 = mix3(at(i0[0], i0[1], i0[2] + db), at(i0[0] + 1, i0[1], i0[2] + db), fr[0]);
+        F3 g1 = mix3(at(i0[0], i0[1] + 1, i0[2] + db), at(i0[0] + 1, i0[1] + 1, i0[2] + db), fr[0]);
+        plane[db] = mix3(g0, g1, fr[1]);
+    }
+    return mix3(plane[0], plane[1], fr[2]);
+}
+uint32_t encode_display(float x) { return uint32_t(clampf(srgb_oetf(x), 0.0f, 1.0f) * 255.0f + 0.5f); }
+
 // display_transform (:37-81) for pixel (x, y): with Uniforms.debug_shadowmaps
 // the top-left quarter shows the four cascades (2 x 2), depth d as
 // (d - 0.5) * 100 + 0.5 without the tone curve (:51-75).
@@ -881,6 +913,10 @@ uint32_t display_pixel(const lh_view& v, uint32_t x, uint32_t y, F3 c) {
             uint32_t e = uint32_t(clampf(srgb_oetf(std::max(g, 0.0f)), 0.0f, 1.0f) * 255.0f + 0.5f);
             return e | (e << 8) | (e << 16) | (255u << 24);
         }
+    }
+    if (v.tonemap_lut) {
+        F3 d = lut_lookup(v.tonemap_lut, c);
+        return encode_display(d.z) | (encode_display(d.y) << 8) | (encode_display(d.x) << 16) | (255u << 24);
     }
     return tonemap_pixel(c);
 }

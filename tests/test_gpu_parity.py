@@ -111,6 +111,35 @@ def test_debug_views_match_oracle(mode):
     print(mode, compare_frames(gpu, orc, scene), flush=True)
 
 
+def test_tonemap_lut_matches_oracle():
+    """A supplied display LUT (lh_view.tonemap_lut, random contents) through
+    the fused resolve + display transform, against the oracle."""
+    import torch
+
+    from lighthugger_amd.lib import Renderer
+    from oracle import lib as oracle_lib
+    from oracle.compare import oracle_view
+
+    scene = HostScene.preset(abi.PRESET_SMALL)
+    cam = preset_camera(abi.PRESET_SMALL)
+    cfg = ViewConfig(cam.width, cam.height, 512)
+    n = abi.TONEMAP_LUT_DIMS
+    lut = np.random.default_rng(3).random((n, n, n, 4), dtype=np.float32)
+    dev = torch.from_numpy(lut).cuda()
+    r = Renderer(scene, cfg)
+    r.set_camera(cam)
+    r.view.c.tonemap_lut = dev.data_ptr()
+    r.frame()
+    torch.cuda.synchronize()
+    gpu = snapshot(r.view)
+    sd, vd = oracle_view(scene, cfg, make_uniforms(cam, scene.num_instances))
+    vd.c.tonemap_lut = lut.ctypes.data
+    oracle_lib().orc_frame(sd.address, vd.address)
+    orc = snapshot(vd)
+    print(compare_frames(gpu, orc, scene), flush=True)
+    assert (gpu["ldr"] != orc["ldr"]).mean() < 1e-3
+
+
 def test_fixtures_match_oracle_and_golden():
     import hashlib
     import json

@@ -87,3 +87,45 @@ def test_shadow_map_overlay():
     grey = (q & 0xFF) == ((q >> 8) & 0xFF)
     assert grey.all() and ((q >> 16) & 0xFF == q & 0xFF).all()
     assert (q == 0xFFFFFFFF).any()  # cleared shadow texels: depth 1.0 -> white
+
+
+def _lut(fn):
+    """LH_TONEMAP_LUT_DIMS^3 RGBA float LUT, red fastest: entry (r, g, b) = fn(r, g, b) / 47."""
+    n = abi.TONEMAP_LUT_DIMS
+    b, g, r = np.meshgrid(np.arange(n), np.arange(n), np.arange(n), indexing="ij")
+    rgb = np.stack(fn(r, g, b), -1).astype(np.float32) / np.float32(n - 1)
+    return np.ascontiguousarray(np.concatenate([rgb, np.ones(rgb.shape[:-1] + (1,), np.float32)], -1))
+
+
+def _lut_frame(lut):
+    from oracle import lib as oracle_lib
+    from oracle.compare import oracle_view, snapshot
+
+    scene = _scene()
+    cam, cfg = fixture_camera(96, 64, position=(0, 0, 2.2))
+    sd, vd = oracle_view(scene, cfg, make_uniforms(cam, scene.num_instances))
+    if lut is not None:
+        vd.c.tonemap_lut = lut.ctypes.data
+    oracle_lib().orc_frame(sd.address, vd.address)
+    return snapshot(vd)
+
+
+def _channels(ldr):
+    return np.stack([(ldr >> 16) & 0xFF, (ldr >> 8) & 0xFF, ldr & 0xFF], -1).astype(np.int64)
+
+
+def test_tonemap_lut_known_answers():
+    """tony_mc_mapface with a supplied LUT (display_transform.comp:5-17): a
+    constant LUT of 0.5 gives sRGB(0.5) everywhere; the identity LUT gives
+    the LUT-free curve x / (x + 1) to one step; a LUT that swaps red and blue
+    swaps the channels (the axis order: red fastest)."""
+    plain = _channels(_lut_frame(None)["ldr"])
+    const = _channels(_lut_frame(_lut(lambda r, g, b: (r * 0 + 23.5, g * 0 + 23.5, b * 0 + 23.5)))["ldr"])
+    x = 0.5
+    srgb = 1.055 * x ** (1 / 2.4) - 0.055
+    assert (const == int(srgb * 255 + 0.5)).all()
+    ident = _channels(_lut_frame(_lut(lambda r, g, b: (r, g, b)))["ldr"])
+    assert np.abs(ident - plain).max() <= 1
+    swap = _channels(_lut_frame(_lut(lambda r, g, b: (b, g, r)))["ldr"])
+    assert np.abs(swap - ident[..., ::-1]).max() <= 1
+    assert (np.abs(ident[..., 0] - ident[..., 2]) > 3).any()  # the image is not grey
