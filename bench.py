@@ -46,6 +46,8 @@ def parse():
     ap.add_argument("--no-atomic-probe", action="store_true", help="skip the atomicMin ceiling measurement")
     ap.add_argument("--quantized-steps", type=int, default=100,
                     help="frames timed on the same workload in the reference's quantized vertex formats (0 = skip)")
+    ap.add_argument("--bc1-steps", type=int, default=100,
+                    help="frames timed on the same workload with BC1-compressed textures (0 = skip)")
     ap.add_argument("--hiz", action="store_true",
                     help="time the frame with two-phase occlusion culling (output is identical either way; the "
                          "other setting is timed too)")
@@ -256,24 +258,25 @@ def diff_bench(steps: int, tri_scale_pct: int = 18, side: int = 1024) -> dict:
             "unsupported_pixels": unsupported}
 
 
-def quantized_bench(preset: int, cfg, cam, steps: int, warmup: int) -> dict:
-    """The same frame with the scene in the reference's vertex formats (u16
-    positions, i8 normals, u16 uvs, u16 remap indices): wall-clock ms per
-    frame over `steps` frames, like the headline number."""
+def variant_bench(preset: int, cfg, cam, steps: int, warmup: int, **scene_kw) -> dict:
+    """The same frame with the scene stored differently (scene_kw of
+    HostScene.preset: the reference's quantized vertex formats, or BC1
+    textures): wall-clock ms per frame over `steps` frames, like the headline
+    number."""
     import torch
 
     from lighthugger_amd import abi
     from lighthugger_amd.lib import Renderer
     from lighthugger_amd.scene import HostScene
 
-    scene = HostScene.preset(preset, vertex_format=abi.VERTEX_FORMAT_QUANTIZED)
+    scene = HostScene.preset(preset, **scene_kw)
     r = Renderer(scene, cfg)
     r.set_camera(cam)
     for _ in range(max(1, warmup)):
         r.frame()
     torch.cuda.synchronize()
     if r.status()[abi.CNT_OVERFLOW]:
-        raise RuntimeError("capacity overflow in the quantized-vertex frame")
+        raise RuntimeError(f"capacity overflow in the frame with {scene_kw}")
     t0 = time.perf_counter()
     for _ in range(steps):
         r.frame()
@@ -282,7 +285,8 @@ def quantized_bench(preset: int, cfg, cam, steps: int, warmup: int) -> dict:
     _, _, vis0 = r.view.prefix(0)
     candidates = int(scene.instance_triangles()[vis0["index"].astype(np.int64)].sum())
     return {"ms_per_frame": round(ms, 4), "gtri_per_s": round(candidates / (ms * 1e-3) / 1e9, 3),
-            "visible_candidate_triangles": candidates, "geometry_bytes": int(scene.geometry.nbytes)}
+            "visible_candidate_triangles": candidates, "geometry_bytes": int(scene.geometry.nbytes),
+            "texel_bytes": int(scene.texels.nbytes)}
 
 
 def batch_profile(scene, n_cameras: int = 64, repeats: int = 3) -> dict:
@@ -500,8 +504,14 @@ def main():
 
     quantized = None
     if args.quantized_steps > 0:
-        quantized = quantized_bench(preset, cfg, cam, args.quantized_steps, args.warmup)
+        quantized = variant_bench(preset, cfg, cam, args.quantized_steps, args.warmup,
+                                  vertex_format=abi.VERTEX_FORMAT_QUANTIZED)
         quantized["geometry_bytes_fp32"] = int(scene.geometry.nbytes)
+        torch.cuda.empty_cache()
+    bc1 = None
+    if args.bc1_steps > 0:
+        bc1 = variant_bench(preset, cfg, cam, args.bc1_steps, args.warmup, texture_format=abi.TEXTURE_BC1)
+        bc1["texel_bytes_rgba8"] = int(scene.texels.nbytes)
         torch.cuda.empty_cache()
 
     batched = None
@@ -558,6 +568,7 @@ def main():
         },
         "cpu_baseline": cpu,
         "quantized_vertices": quantized,
+        "bc1_textures": bc1,
         "batched_views": batched,
         "differentiable_resolve": differentiable,
     }

@@ -49,6 +49,7 @@ typedef struct lh_preset_config {
     uint32_t texture_size;    /* 0 = preset default (1024 for LARGE/STRESS) */
     uint32_t mesh_tri_scale_pct; /* 0 = 100; scales per-mesh triangle targets (CPU-sized tests) */
     uint32_t vertex_format;   /* LH_VERTEX_FORMAT_* (0 = FP32) */
+    uint32_t texture_format;  /* LH_TEXTURE_RGBA8 or LH_TEXTURE_BC1 (shared_cpu_gpu.h) */
 } lh_preset_config;
 
 /* Camera of one view.  `target` is a point on the view axis; up is +Y. */
@@ -87,6 +88,10 @@ typedef struct lh_scene_arrays {
 
 /* ---- incremental builder (hand-built fixtures and presets) ---- */
 lh_scene_host* lh_scene_create(void);
+/* Texture storage of the finished scene: LH_TEXTURE_RGBA8 (default) or
+ * LH_TEXTURE_BC1 (each mip level encoded to BC1 blocks, decoded on fetch by
+ * the kernels; alpha dropped).  Before lh_scene_finish.  0 on success. */
+int32_t lh_scene_set_texture_format(lh_scene_host* s, uint32_t format);
 /* Selects the vertex format (LH_VERTEX_FORMAT_*) of the meshes added after
  * it; only before the first mesh.  0 on success. */
 int32_t lh_scene_set_vertex_format(lh_scene_host* s, uint32_t format);

@@ -176,15 +176,21 @@ typedef struct LhMaterialFactors {
     float metallic_factor;
 } LhMaterialFactors;
 
-/* One RGBA8 UNORM texture with a full mip chain packed level after level. */
+/* One texture with a full mip chain packed level after level, RGBA8 UNORM or
+ * BC1 blocks (decoded on fetch).
+ *   LH_TEXTURE_RGBA8: 4 bytes per texel, row-major; level_offset in texels.
+ *   LH_TEXTURE_BC1:   8-byte blocks of 4x4 texels, ceil(w/4) blocks per row;
+ *                     level_offset in blocks. */
 #define LH_MAX_TEXTURE_LEVELS 16u
+#define LH_TEXTURE_RGBA8 0u
+#define LH_TEXTURE_BC1 1u
 typedef struct LhTexture {
-    uint64_t texels; /* device address of level 0; level l at texels + 4*level_offset[l] */
+    uint64_t texels; /* device address of level 0 */
     uint32_t width;
     uint32_t height;
     uint32_t levels;
-    uint32_t pad;
-    uint32_t level_offset[LH_MAX_TEXTURE_LEVELS]; /* in texels */
+    uint32_t format; /* LH_TEXTURE_* */
+    uint32_t level_offset[LH_MAX_TEXTURE_LEVELS];
 } LhTexture;
 
 /* Visibility-buffer key: depth_key << 32 | meshlet_id << 7 | triangle.

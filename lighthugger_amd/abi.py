@@ -14,6 +14,7 @@ MESH_INFO_FLAGS_ALPHA_CLIP = 0x02
 MESH_INFO_FLAGS_FP32_VERTICES = 0x04
 SCENE_OPAQUE_ONLY = 0x1  # LH_SCENE_OPAQUE_ONLY
 TONEMAP_LUT_DIMS = 48  # LH_TONEMAP_LUT_DIMS
+TEXTURE_RGBA8, TEXTURE_BC1 = 0, 1  # LhTexture.format
 # Uniforms.debug (src/shared_cpu_gpu.h:46-51)
 (UNIFORMS_DEBUG_OFF, UNIFORMS_DEBUG_CASCADES, UNIFORMS_DEBUG_TRIANGLE_INDEX, UNIFORMS_DEBUG_INSTANCE_INDEX,
  UNIFORMS_DEBUG_SHADER_CLOCK, UNIFORMS_DEBUG_NORMALS) = range(6)
@@ -60,7 +61,7 @@ MESHLET_REFERENCE = _dt([("instance_index", "<u4", 0), ("meshlet_index", "<u2", 
 PREFIX_SUM_VALUE = _dt([("index", "<u4", 0), ("sum", "<u4", 4)], 8)
 MATERIAL_FACTORS = _dt([("roughness_factor", "<f4", 0), ("metallic_factor", "<f4", 4)], 8)
 TEXTURE = _dt(
-    [("texels", "<u8", 0), ("width", "<u4", 8), ("height", "<u4", 12), ("levels", "<u4", 16), ("pad", "<u4", 20),
+    [("texels", "<u8", 0), ("width", "<u4", 8), ("height", "<u4", 12), ("levels", "<u4", 16), ("format", "<u4", 20),
      ("level_offset", ("<u4", 16), 24)],
     88,
 )
@@ -110,7 +111,8 @@ class Camera(C.Structure):
 
 class PresetConfig(C.Structure):
     _fields_ = [("preset", C.c_uint32), ("num_instances", C.c_uint32), ("seed", C.c_uint64),
-                ("texture_size", C.c_uint32), ("mesh_tri_scale_pct", C.c_uint32), ("vertex_format", C.c_uint32)]
+                ("texture_size", C.c_uint32), ("mesh_tri_scale_pct", C.c_uint32), ("vertex_format", C.c_uint32),
+                ("texture_format", C.c_uint32)]
 
 
 class SceneArrays(C.Structure):

@@ -156,7 +156,7 @@ LH_DEV f4 texel(const LhTexture& t, uint32_t level, int32_t w, int32_t h, int32_
     if (i < 0) i += w;
     j %= h;
     if (j < 0) j += h;
-    const uint32_t q = ptr<uint32_t>(t.texels)[size_t(t.level_offset[level]) + size_t(j) * w + i];
+    const uint32_t q = texel_rgba8(t, level, uint32_t(w), uint32_t(i), uint32_t(j));
     return {float(q & 255u) / 255.0f, float((q >> 8) & 255u) / 255.0f, float((q >> 16) & 255u) / 255.0f,
             float(q >> 24) / 255.0f};
 }

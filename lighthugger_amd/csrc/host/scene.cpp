@@ -552,6 +552,69 @@ struct TextureData {
     std::vector<uint8_t> rgba;           // all levels
 };
 
+// BC1 (S3TC) encoder of the producer.  Per 4x4 block the endpoints are the
+// texels of largest and smallest luma (r + 2g + b), rounded to RGB565 and
+// ordered c0 > c1 (four-colour mode); every texel takes the palette entry, as
+// the decoder forms it (endpoints widened by bit replication, thirds rounded
+// to nearest), nearest in squared RGB distance.  Texels beyond the level's
+// edge repeat the edge; alpha is dropped (opaque).
+uint16_t to565(const uint8_t* p) {
+    return uint16_t((((p[0] * 31u + 127u) / 255u) << 11) | (((p[1] * 63u + 127u) / 255u) << 5) |
+                    ((p[2] * 31u + 127u) / 255u));
+}
+void bc1_palette(uint16_t c0, uint16_t c1, int pal[4][3]) {
+    auto widen = [](uint32_t v, int o[3]) {
+        uint32_t r = (v >> 11) & 31u, g = (v >> 5) & 63u, b = v & 31u;
+        o[0] = int((r << 3) | (r >> 2));
+        o[1] = int((g << 2) | (g >> 4));
+        o[2] = int((b << 3) | (b >> 2));
+    };
+    widen(c0, pal[0]);
+    widen(c1, pal[1]);
+    for (int k = 0; k < 3; k++) {
+        pal[2][k] = (2 * pal[0][k] + pal[1][k] + 1) / 3;
+        pal[3][k] = (pal[0][k] + 2 * pal[1][k] + 1) / 3;
+    }
+}
+void encode_bc1_level(const uint8_t* rgba, uint32_t w, uint32_t h, std::vector<uint8_t>& out) {
+    for (uint32_t by = 0; by < (h + 3) / 4; by++)
+        for (uint32_t bx = 0; bx < (w + 3) / 4; bx++) {
+            const uint8_t* px[16];
+            for (uint32_t k = 0; k < 16; k++) {
+                uint32_t x = std::min(w - 1, bx * 4 + (k & 3)), y = std::min(h - 1, by * 4 + (k >> 2));
+                px[k] = rgba + 4 * (size_t(y) * w + x);
+            }
+            int lo = 0, hi = 0;
+            auto luma = [&](int k) { return int(px[k][0]) + 2 * int(px[k][1]) + int(px[k][2]); };
+            for (int k = 1; k < 16; k++) {
+                if (luma(k) < luma(lo)) lo = k;
+                if (luma(k) > luma(hi)) hi = k;
+            }
+            uint16_t c0 = to565(px[hi]), c1 = to565(px[lo]);
+            if (c0 < c1) std::swap(c0, c1);
+            uint32_t idx = 0;
+            if (c0 != c1) {
+                int pal[4][3];
+                bc1_palette(c0, c1, pal);
+                for (int k = 0; k < 16; k++) {
+                    int best = 0, bd = 1 << 30;
+                    for (int s = 0; s < 4; s++) {
+                        int d = 0;
+                        for (int c = 0; c < 3; c++) d += (int(px[k][c]) - pal[s][c]) * (int(px[k][c]) - pal[s][c]);
+                        if (d < bd) {
+                            bd = d;
+                            best = s;
+                        }
+                    }
+                    idx |= uint32_t(best) << (2 * k);
+                }
+            }
+            const uint8_t blk[8] = {uint8_t(c0), uint8_t(c0 >> 8), uint8_t(c1), uint8_t(c1 >> 8),
+                                    uint8_t(idx), uint8_t(idx >> 8), uint8_t(idx >> 16), uint8_t(idx >> 24)};
+            out.insert(out.end(), blk, blk + 8);
+        }
+}
+
 void build_mips(TextureData& t) {
     uint32_t w = t.w, h = t.h;
     t.level_offset = {0};
@@ -724,6 +787,7 @@ LhMat4 mul4(const LhMat4& a, const LhMat4& b) {
 
 struct lh_scene_host {
     uint32_t vertex_format = LH_VERTEX_FORMAT_FP32;
+    uint32_t texture_format = LH_TEXTURE_RGBA8;
     std::vector<std::unique_ptr<BuiltMesh>> meshes;
     std::vector<TextureData> textures;
     std::vector<MaterialSpec> materials;
@@ -767,6 +831,12 @@ int32_t add_mesh(lh_scene_host* s, MeshSource&& src) {
 extern "C" {
 
 lh_scene_host* lh_scene_create(void) { return new lh_scene_host(); }
+
+int32_t lh_scene_set_texture_format(lh_scene_host* s, uint32_t format) {
+    if (!s || s->finished || format > LH_TEXTURE_BC1) return -1;
+    s->texture_format = format;
+    return 0;
+}
 
 int32_t lh_scene_set_vertex_format(lh_scene_host* s, uint32_t format) {
     if (!s || s->finished || !s->meshes.empty() || format > LH_VERTEX_FORMAT_QUANTIZED) return -1;
@@ -937,12 +1007,22 @@ int32_t lh_scene_finish(lh_scene_host* s) {
     // Texture arena.
     for (const auto& t : s->textures) {
         LhTexture lt{};
-        uint64_t off = append_bytes(s->texels, t.rgba);
-        lt.texels = off;
         lt.width = t.w;
         lt.height = t.h;
         lt.levels = t.levels;
-        for (uint32_t l = 0; l < t.levels; l++) lt.level_offset[l] = t.level_offset[l];
+        lt.format = s->texture_format;
+        if (s->texture_format == LH_TEXTURE_BC1) {
+            std::vector<uint8_t> blocks;
+            for (uint32_t l = 0; l < t.levels; l++) {
+                lt.level_offset[l] = uint32_t(blocks.size() / 8);
+                const uint32_t w = std::max(1u, t.w >> l), h = std::max(1u, t.h >> l);
+                encode_bc1_level(t.rgba.data() + size_t(t.level_offset[l]) * 4, w, h, blocks);
+            }
+            lt.texels = append_bytes(s->texels, blocks);
+        } else {
+            lt.texels = append_bytes(s->texels, t.rgba);
+            for (uint32_t l = 0; l < t.levels; l++) lt.level_offset[l] = t.level_offset[l];
+        }
         s->tex_out.push_back(lt);
     }
     if (s->texels.empty()) s->texels.resize(256);
@@ -1114,6 +1194,7 @@ lh_scene_host* lh_scene_build_preset(const lh_preset_config* cfg) {
     Rng rng(seed);
     auto s = std::make_unique<lh_scene_host>();
     if (lh_scene_set_vertex_format(s.get(), cfg->vertex_format) != 0) return nullptr;
+    if (lh_scene_set_texture_format(s.get(), cfg->texture_format) != 0) return nullptr;
     const float tri_scale = (cfg->mesh_tri_scale_pct ? cfg->mesh_tri_scale_pct : 100) / 100.0f;
     if (cfg->preset == LH_PRESET_SMALL) {
         // One sphere of exactly 8,192 triangles (octahedron, level 5), 4 constant

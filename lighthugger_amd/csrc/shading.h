@@ -105,13 +105,41 @@ LH_DEV Interp2 interp2(const BaryDeriv& d, f2 a, f2 b, f2 c) {
 
 LH_DEV f4 mix4(f4 a, f4 b, float t) { return {mixf(a.x, b.x, t), mixf(a.y, b.y, t), mixf(a.z, b.z, t), mixf(a.w, b.w, t)}; }
 
+// BC1 (S3TC) colour of index `sel` of a block with RGB565 endpoints c0, c1:
+// endpoints widened by bit replication; c0 > c1: c0, c1, (2 c0 + c1 + 1) / 3,
+// (c0 + 2 c1 + 1) / 3, else c0, c1, (c0 + c1 + 1) / 2 and transparent black.
+// Packed RGBA8, red in the low byte.
+LH_DEV uint32_t widen565(uint32_t c) {
+    const uint32_t r = (c >> 11) & 31u, g = (c >> 5) & 63u, b = c & 31u;
+    return ((r << 3) | (r >> 2)) | (((g << 2) | (g >> 4)) << 8) | (((b << 3) | (b >> 2)) << 16);
+}
+LH_DEV uint32_t bc1_colour(uint32_t c0, uint32_t c1, uint32_t sel) {
+    const uint32_t a = widen565(c0), b = widen565(c1);
+    if (sel == 0) return a | 0xFF000000u;
+    if (sel == 1) return b | 0xFF000000u;
+    if (c0 <= c1 && sel == 3) return 0u;
+    uint32_t out = 0xFF000000u;
+    for (uint32_t k = 0; k < 24; k += 8) {
+        const uint32_t x = (a >> k) & 255u, y = (b >> k) & 255u;
+        const uint32_t m = c0 > c1 ? (sel == 2 ? (2 * x + y + 1) / 3 : (x + 2 * y + 1) / 3) : (x + y + 1) / 2;
+        out |= m << k;
+    }
+    return out;
+}
+// Texel (i, j) of a level of width w (already wrapped), packed RGBA8.
+LH_DEV uint32_t texel_rgba8(const LhTexture& t, uint32_t level, uint32_t w, uint32_t i, uint32_t j) {
+    if (t.format != LH_TEXTURE_BC1) return ptr<uint32_t>(t.texels)[size_t(t.level_offset[level]) + size_t(j) * w + i];
+    const uint2 blk = ptr<uint2>(t.texels)[size_t(t.level_offset[level]) + size_t(j >> 2) * ((w + 3) >> 2) + (i >> 2)];
+    return bc1_colour(blk.x & 0xFFFFu, blk.x >> 16, (blk.y >> (2 * (4 * (j & 3) + (i & 3)))) & 3u);
+}
+
 LH_DEV f4 texel(const LhTexture& t, uint32_t level, int32_t i, int32_t j) {
     const int32_t w = int32_t(max(1u, t.width >> level)), h = int32_t(max(1u, t.height >> level));
     i %= w;
     if (i < 0) i += w;
     j %= h;
     if (j < 0) j += h;
-    const uint32_t p = ptr<uint32_t>(t.texels)[size_t(t.level_offset[level]) + size_t(j) * w + i];
+    const uint32_t p = texel_rgba8(t, level, uint32_t(w), uint32_t(i), uint32_t(j));
     return {float(p & 255u) / 255.0f, float((p >> 8) & 255u) / 255.0f, float((p >> 16) & 255u) / 255.0f,
             float(p >> 24) / 255.0f};
 }

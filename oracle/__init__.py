@@ -52,6 +52,8 @@ def lib() -> C.CDLL:
         L.orc_srgb_oetf.argtypes = [C.c_float]
         L.orc_frame_threads.restype = C.c_int32
         L.orc_frame_threads.argtypes = [C.c_void_p, C.c_void_p, C.c_uint32]
+        L.orc_bc1_texel.restype = C.c_uint32
+        L.orc_bc1_texel.argtypes = [C.c_void_p, C.c_uint32, C.c_uint32]
         L.orc_hiz_bound_violations.restype = C.c_int64
         L.orc_hiz_bound_violations.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
         L.orc_resolve_frozen.restype = C.c_int32

@@ -10,6 +10,7 @@
 // compiled with -ffp-contract=off; no transcendental function feeds a
 // bit-exact result (powers of the cascade split are integer powers).
 #include "oracle.h"
+#include "texel.h"
 
 #include <algorithm>
 #include <atomic>
@@ -562,8 +563,9 @@ F4 texel(const LhTexture& t, uint32_t level, int32_t i, int32_t j) {
     if (i < 0) i += w;
     j %= h;
     if (j < 0) j += h;
-    const uint8_t* p = at<uint8_t>(t.texels) + 4 * (size_t(t.level_offset[level]) + size_t(j) * w + i);
-    return {p[0] / 255.0f, p[1] / 255.0f, p[2] / 255.0f, p[3] / 255.0f};
+    const orc_texel::RGBA8 c =
+        orc_texel::fetch(t, at<uint8_t>(t.texels), level, uint32_t(w), uint32_t(i), uint32_t(j));
+    return {c.c[0] / 255.0f, c.c[1] / 255.0f, c.c[2] / 255.0f, c.c[3] / 255.0f};
 }
 F4 mix4(F4 a, F4 b, float t) { return {mixf(a.x, b.x, t), mixf(a.y, b.y, t), mixf(a.z, b.z, t), mixf(a.w, b.w, t)}; }
 F4 bilinear(const LhTexture& t, uint32_t level, F2 uv) {
@@ -1012,6 +1014,15 @@ bool sphere_screen_bound(const LhMat4& m, F3 c, float r, uint32_t W, uint32_t H,
 }  // namespace
 
 extern "C" {
+
+// One texel of a BC1 block (8 bytes), packed RGBA8 with red in the low byte.
+uint32_t orc_bc1_texel(const uint8_t* block, uint32_t x, uint32_t y) {
+    LhTexture t{};
+    t.format = LH_TEXTURE_BC1;
+    const orc_texel::RGBA8 c = orc_texel::fetch(t, block, 0, 4, x & 3u, y & 3u);
+    return uint32_t(c.c[0]) | (uint32_t(c.c[1]) << 8) | (uint32_t(c.c[2]) << 16) | (uint32_t(c.c[3]) << 24);
+}
+
 
 int64_t orc_hiz_bound_violations(const lh_scene* sc, const lh_view* v, uint64_t* checked) {
     if (!sc || !v || !checked) return -1;

@@ -71,6 +71,8 @@ float orc_srgb_oetf(float x);
  * exceed the bound's depth (1e-4 relative slack).  Returns the number of
  * violations; *checked counts the triangles examined. */
 int64_t orc_hiz_bound_violations(const lh_scene* scene, const lh_view* view, uint64_t* checked);
+/* One texel (x, y) of a BC1 block (8 bytes), packed RGBA8 with red in the low byte. */
+uint32_t orc_bc1_texel(const uint8_t* block, uint32_t x, uint32_t y);
 
 /* ---- differentiable resolve ----
  * Per-pixel quantities the adjoint holds fixed (FP32, as the forward resolve

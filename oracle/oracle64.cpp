@@ -15,6 +15,8 @@
 #include <cmath>
 #include <cstring>
 
+#include "texel.h"
+
 #include "oracle.h"
 
 namespace {
@@ -193,7 +195,9 @@ V4<double> texel(const LhTexture& t, uint32_t level, int64_t i, int64_t j) {
     if (i < 0) i += w;
     j %= h;
     if (j < 0) j += h;
-    const uint8_t* q = at<uint8_t>(t.texels) + 4 * (size_t(t.level_offset[level]) + size_t(j) * w + i);
+    const orc_texel::RGBA8 c =
+        orc_texel::fetch(t, at<uint8_t>(t.texels), level, uint32_t(w), uint32_t(i), uint32_t(j));
+    const uint8_t* q = c.c;
     return {q[0] / 255.0, q[1] / 255.0, q[2] / 255.0, q[3] / 255.0};
 }
 template <class T>

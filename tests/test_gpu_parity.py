@@ -140,6 +140,15 @@ def test_tonemap_lut_matches_oracle():
     assert (gpu["ldr"] != orc["ldr"]).mean() < 1e-3
 
 
+def test_bc1_texture_frame_parity():
+    """The benchmark workload at full size with every texture in BC1, decoded
+    on fetch by the resolve: keys bit-exact, radiance at the usual bar."""
+    scene = HostScene.preset(abi.PRESET_LARGE, texture_format=abi.TEXTURE_BC1)
+    cam = preset_camera(abi.PRESET_LARGE)
+    rep = _run(scene, cam, ViewConfig(cam.width, cam.height, 2048, big_capacity=1 << 21))
+    assert rep["covered_pixels"] > 100000
+
+
 def test_fixtures_match_oracle_and_golden():
     import hashlib
     import json
