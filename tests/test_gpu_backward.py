@@ -22,13 +22,14 @@ pytestmark = pytest.mark.gpu
 REL_L2 = 1e-4
 
 
-def _setup(width=256, height=192, shadow=512, tri_scale_pct=10, vertex_format=abi.VERTEX_FORMAT_FP32):
+def _setup(width=256, height=192, shadow=512, tri_scale_pct=10, vertex_format=abi.VERTEX_FORMAT_FP32,
+           texture_format=abi.TEXTURE_RGBA8):
     import torch
 
     from lighthugger_amd.lib import Renderer
 
     scene = HostScene.preset(abi.PRESET_LARGE, texture_size=64, tri_scale_pct=tri_scale_pct,
-                             vertex_format=vertex_format)
+                             vertex_format=vertex_format, texture_format=texture_format)
     base = preset_camera(abi.PRESET_LARGE)
     cam = make_camera(base.position.tolist(), base.target.tolist(), width, height)
     cfg = ViewConfig(width, height, shadow, big_capacity=1 << 20)
@@ -79,12 +80,13 @@ def test_resolve_ex_with_scene_values_is_lh_resolve():
     assert np.array_equal(out, ref)
 
 
-def test_backward_matches_fp64_adjoint():
+@pytest.mark.parametrize("texture_format", [abi.TEXTURE_RGBA8, abi.TEXTURE_BC1])
+def test_backward_matches_fp64_adjoint(texture_format):
     import torch
 
     from lighthugger_amd.diff import DifferentiableResolve
 
-    scene, r, sd, vd = _setup()
+    scene, r, sd, vd = _setup(texture_format=texture_format)
     d = DifferentiableResolve(r)
     p = d.parameters()
     rng = np.random.default_rng(5)
