@@ -50,6 +50,7 @@ typedef struct lh_preset_config {
     uint32_t mesh_tri_scale_pct; /* 0 = 100; scales per-mesh triangle targets (CPU-sized tests) */
     uint32_t vertex_format;   /* LH_VERTEX_FORMAT_* (0 = FP32) */
     uint32_t texture_format;  /* LH_TEXTURE_RGBA8 or LH_TEXTURE_BC1 (shared_cpu_gpu.h) */
+    const char* cache_dir;    /* NULL, or an existing directory for the built-mesh cache */
 } lh_preset_config;
 
 /* Camera of one view.  `target` is a point on the view axis; up is +Y. */
@@ -88,6 +89,11 @@ typedef struct lh_scene_arrays {
 
 /* ---- incremental builder (hand-built fixtures and presets) ---- */
 lh_scene_host* lh_scene_create(void);
+/* On-disk cache of built meshes (the reference's FsCache): meshes added
+ * afterwards are looked up in `dir` by a hash of their source arrays and the
+ * vertex format, and stored there when built.  `dir` must exist; NULL or ""
+ * switches the cache off.  Only before the first mesh.  0 on success. */
+int32_t lh_scene_set_cache_dir(lh_scene_host* s, const char* dir);
 /* Texture storage of the finished scene: LH_TEXTURE_RGBA8 (default) or
  * LH_TEXTURE_BC1 (each mip level encoded to BC1 blocks, decoded on fetch by
  * the kernels; alpha dropped).  Before lh_scene_finish.  0 on success. */

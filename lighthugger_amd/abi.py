@@ -112,7 +112,7 @@ class Camera(C.Structure):
 class PresetConfig(C.Structure):
     _fields_ = [("preset", C.c_uint32), ("num_instances", C.c_uint32), ("seed", C.c_uint64),
                 ("texture_size", C.c_uint32), ("mesh_tri_scale_pct", C.c_uint32), ("vertex_format", C.c_uint32),
-                ("texture_format", C.c_uint32)]
+                ("texture_format", C.c_uint32), ("cache_dir", C.c_char_p)]
 
 
 class SceneArrays(C.Structure):

@@ -21,8 +21,10 @@
 #include <algorithm>
 #include <array>
 #include <cmath>
+#include <cstdio>
 #include <cstring>
 #include <memory>
+#include <string>
 #include <numeric>
 #include <thread>
 #include <unordered_map>
@@ -786,6 +788,7 @@ LhMat4 mul4(const LhMat4& a, const LhMat4& b) {
 // ------------------------------------------------------------------ scene
 
 struct lh_scene_host {
+    std::string cache_dir;  // on-disk cache of built meshes; empty = none
     uint32_t vertex_format = LH_VERTEX_FORMAT_FP32;
     uint32_t texture_format = LH_TEXTURE_RGBA8;
     std::vector<std::unique_ptr<BuiltMesh>> meshes;
@@ -815,12 +818,143 @@ uint64_t append_bytes(std::vector<uint8_t>& arena, const std::vector<T>& v) {
     return off;
 }
 
+// ---- on-disk cache of built meshes (the reference's FsCache,
+// src/resources/fs_cache.h:3-25, which stores built meshlets per asset).
+// Key: FNV-1a 64 of the source arrays, the vertex format and the file
+// version; file: <dir>/lhmesh-<key>.bin holding the built mesh (reordered
+// vertices, meshlets, remap, micro indices, bounds, quantization grid).  A
+// file whose checksum or sizes do not hold is ignored (the mesh is rebuilt).
+constexpr uint32_t kCacheMagic = 0x434D484Cu;  // "LHMC"
+constexpr uint32_t kCacheVersion = 1;
+
+uint64_t fnv1a(const void* data, size_t n, uint64_t h = 1469598103934665603ull) {
+    const uint8_t* p = static_cast<const uint8_t*>(data);
+    for (size_t i = 0; i < n; i++) h = (h ^ p[i]) * 1099511628211ull;
+    return h;
+}
+template <class T>
+uint64_t fnv1a_vec(const std::vector<T>& v, uint64_t h) {
+    const uint64_t n = v.size();
+    h = fnv1a(&n, sizeof n, h);
+    return v.empty() ? h : fnv1a(v.data(), v.size() * sizeof(T), h);
+}
+std::string cache_path(const lh_scene_host* s, const MeshSource& m) {
+    uint64_t h = fnv1a(&kCacheVersion, sizeof kCacheVersion);
+    h = fnv1a(&s->vertex_format, sizeof s->vertex_format, h);
+    h = fnv1a_vec(m.pos, h);
+    h = fnv1a_vec(m.nrm, h);
+    h = fnv1a_vec(m.uv, h);
+    h = fnv1a_vec(m.idx, h);
+    char name[64];
+    std::snprintf(name, sizeof name, "/lhmesh-%016llx.bin", static_cast<unsigned long long>(h));
+    return s->cache_dir + name;
+}
+// Files: magic, version, the arrays (u64 count + elements), bounds and grid,
+// then an FNV-1a 64 of everything before it.
+template <class T>
+void put(std::vector<uint8_t>& out, const T* p, size_t n) {
+    const uint8_t* b = reinterpret_cast<const uint8_t*>(p);
+    out.insert(out.end(), b, b + n * sizeof(T));
+}
+template <class T>
+void put_vec(std::vector<uint8_t>& out, const std::vector<T>& v) {
+    const uint64_t n = v.size();
+    put(out, &n, 1);
+    put(out, v.data(), v.size());
+}
+struct Reader {
+    const std::vector<uint8_t>& b;
+    size_t at = 0;
+    template <class T>
+    bool get(T* p, size_t n) {
+        if (n * sizeof(T) > b.size() - at) return false;
+        std::memcpy(p, b.data() + at, n * sizeof(T));
+        at += n * sizeof(T);
+        return true;
+    }
+    template <class T>
+    bool get_vec(std::vector<T>& v, uint64_t max_n) {
+        uint64_t n = 0;
+        if (!get(&n, 1) || n > max_n) return false;
+        v.resize(n);
+        return get(v.data(), n);
+    }
+};
+void save_cached(const std::string& path, const BuiltMesh& bm) {
+    std::vector<uint8_t> out;
+    const uint32_t head[2] = {kCacheMagic, kCacheVersion};
+    put(out, head, 2);
+    put_vec(out, bm.src.pos);
+    put_vec(out, bm.src.nrm);
+    put_vec(out, bm.src.uv);
+    put_vec(out, bm.src.idx);
+    put_vec(out, bm.meshlets);
+    put_vec(out, bm.remap);
+    put_vec(out, bm.micro);
+    const uint8_t q = bm.quantized ? 1 : 0;
+    put(out, bm.sphere.data(), 4);
+    put(out, &q, 1);
+    put(out, bm.grid_lo, 3);
+    put(out, &bm.step, 1);
+    const uint64_t h = fnv1a(out.data(), out.size());
+    put(out, &h, 1);
+    const std::string tmp = path + ".tmp";
+    std::FILE* f = std::fopen(tmp.c_str(), "wb");
+    if (!f) return;
+    const bool ok = std::fwrite(out.data(), 1, out.size(), f) == out.size();
+    if (std::fclose(f) == 0 && ok)
+        std::rename(tmp.c_str(), path.c_str());
+    else
+        std::remove(tmp.c_str());
+}
+bool load_cached(const std::string& path, BuiltMesh& bm, const MeshSource& src) {
+    std::FILE* f = std::fopen(path.c_str(), "rb");
+    if (!f) return false;
+    std::vector<uint8_t> buf;
+    uint8_t chunk[1 << 16];
+    size_t n;
+    while ((n = std::fread(chunk, 1, sizeof chunk, f)) > 0) buf.insert(buf.end(), chunk, chunk + n);
+    std::fclose(f);
+    if (buf.size() < 16) return false;
+    uint64_t h = 0;
+    std::memcpy(&h, buf.data() + buf.size() - 8, 8);
+    if (h != fnv1a(buf.data(), buf.size() - 8)) return false;  // damaged or truncated
+    buf.resize(buf.size() - 8);
+    BuiltMesh c;
+    Reader r{buf};
+    uint32_t head[2] = {0, 0};
+    uint8_t q = 0;
+    const uint64_t nv = src.nv(), nt = src.nt();
+    bool ok = r.get(head, 2) && head[0] == kCacheMagic && head[1] == kCacheVersion && r.get_vec(c.src.pos, 3 * nv) &&
+              r.get_vec(c.src.nrm, 3 * nv) && r.get_vec(c.src.uv, 2 * nv) && r.get_vec(c.src.idx, 3 * nt) &&
+              r.get_vec(c.meshlets, nt) && r.get_vec(c.remap, 64 * nt + 64) && r.get_vec(c.micro, 8 * nt) &&
+              r.get(c.sphere.data(), 4) && r.get(&q, 1) && r.get(c.grid_lo, 3) && r.get(&c.step, 1) &&
+              r.at == buf.size();
+    ok = ok && c.src.pos.size() == 3 * nv && c.src.nrm.size() == 3 * nv && c.src.uv.size() == 2 * nv &&
+         c.src.idx.size() == 3 * nt && !c.meshlets.empty();
+    // every meshlet's ranges within the stored arrays, every remap entry a vertex
+    for (size_t k = 0; ok && k < c.meshlets.size(); k++) {
+        const Meshlet& ml = c.meshlets[k];
+        ok = size_t(ml.index_offset) + ml.index_count <= c.remap.size() &&
+             size_t(ml.triangle_offset) + 3u * ml.triangle_count <= c.micro.size() && ml.triangle_count > 0;
+    }
+    for (size_t k = 0; ok && k < c.remap.size(); k++) ok = c.remap[k] < nv;
+    if (!ok) return false;
+    c.quantized = q != 0;
+    bm = std::move(c);
+    return true;
+}
+
 int32_t add_mesh(lh_scene_host* s, MeshSource&& src) {
     if (s->finished || src.nt() == 0) return -1;
     auto bm = std::make_unique<BuiltMesh>();
-    bm->src = std::move(src);
-    if (s->vertex_format == LH_VERTEX_FORMAT_QUANTIZED) quantize_positions(*bm);
-    build_meshlets(*bm);
+    const std::string path = s->cache_dir.empty() ? std::string() : cache_path(s, src);
+    if (path.empty() || !load_cached(path, *bm, src)) {
+        bm->src = std::move(src);
+        if (s->vertex_format == LH_VERTEX_FORMAT_QUANTIZED) quantize_positions(*bm);
+        build_meshlets(*bm);
+        if (!path.empty()) save_cached(path, *bm);
+    }
     if (bm->meshlets.size() >= (1u << 16)) return -1;  // MeshInfo.num_meshlets is u16
     s->meshes.push_back(std::move(bm));
     return int32_t(s->meshes.size() - 1);
@@ -831,6 +965,12 @@ int32_t add_mesh(lh_scene_host* s, MeshSource&& src) {
 extern "C" {
 
 lh_scene_host* lh_scene_create(void) { return new lh_scene_host(); }
+
+int32_t lh_scene_set_cache_dir(lh_scene_host* s, const char* dir) {
+    if (!s || s->finished || !s->meshes.empty()) return -1;
+    s->cache_dir = dir ? dir : "";
+    return 0;
+}
 
 int32_t lh_scene_set_texture_format(lh_scene_host* s, uint32_t format) {
     if (!s || s->finished || format > LH_TEXTURE_BC1) return -1;
@@ -1195,6 +1335,7 @@ lh_scene_host* lh_scene_build_preset(const lh_preset_config* cfg) {
     auto s = std::make_unique<lh_scene_host>();
     if (lh_scene_set_vertex_format(s.get(), cfg->vertex_format) != 0) return nullptr;
     if (lh_scene_set_texture_format(s.get(), cfg->texture_format) != 0) return nullptr;
+    if (cfg->cache_dir && lh_scene_set_cache_dir(s.get(), cfg->cache_dir) != 0) return nullptr;
     const float tri_scale = (cfg->mesh_tri_scale_pct ? cfg->mesh_tri_scale_pct : 100) / 100.0f;
     if (cfg->preset == LH_PRESET_SMALL) {
         // One sphere of exactly 8,192 triangles (octahedron, level 5), 4 constant

@@ -157,8 +157,10 @@ class HostScene:
 
     @classmethod
     def preset(cls, preset: int, *, num_instances=0, seed=0, texture_size=0, tri_scale_pct=0,
-               vertex_format=abi.VERTEX_FORMAT_FP32, texture_format=abi.TEXTURE_RGBA8) -> "HostScene":
-        cfg = abi.PresetConfig(preset, num_instances, seed, texture_size, tri_scale_pct, vertex_format, texture_format)
+               vertex_format=abi.VERTEX_FORMAT_FP32, texture_format=abi.TEXTURE_RGBA8, cache_dir=None) -> "HostScene":
+        """cache_dir: an existing directory for the built-mesh cache (None: no cache)."""
+        cfg = abi.PresetConfig(preset, num_instances, seed, texture_size, tri_scale_pct, vertex_format, texture_format,
+                               None if cache_dir is None else str(cache_dir).encode())
         return cls(scene_lib().lh_scene_build_preset(C.byref(cfg)))
 
     @classmethod

@@ -227,3 +227,30 @@ def test_vertex_format_rejected_after_first_mesh():
         assert lib.lh_scene_set_vertex_format(h, abi.VERTEX_FORMAT_FP32) != 0
     finally:
         lib.lh_scene_destroy(h)
+
+
+@pytest.mark.parametrize("vertex_format", [abi.VERTEX_FORMAT_FP32, abi.VERTEX_FORMAT_QUANTIZED])
+def test_built_mesh_cache(tmp_path, vertex_format):
+    """The on-disk cache of built meshes (the reference's FsCache): a second
+    build reads every mesh back and gives the same scene; a damaged file is
+    ignored and rebuilt; the vertex format is part of the key."""
+    kw = dict(tri_scale_pct=5, texture_size=16, vertex_format=vertex_format)
+    plain = HostScene.preset(abi.PRESET_LARGE, **kw)
+    first = HostScene.preset(abi.PRESET_LARGE, cache_dir=tmp_path, **kw)
+    files = sorted(tmp_path.glob("lhmesh-*.bin"))
+    # content-addressed: meshes with identical sources (some boxes) share a file
+    assert 0.8 * plain.num_meshes <= len(files) <= plain.num_meshes
+    second = HostScene.preset(abi.PRESET_LARGE, cache_dir=tmp_path, **kw)
+    for s in (first, second):
+        for k in ("geometry", "mesh_infos", "instances", "meshlet_table", "texels"):
+            assert np.array_equal(getattr(s, k), getattr(plain, k)), k
+    # damage two files: truncated, and a remap entry out of range
+    files[0].write_bytes(files[0].read_bytes()[:100])
+    raw = bytearray(files[1].read_bytes())
+    raw[-30:-20] = b"\xff" * 10
+    files[1].write_bytes(bytes(raw))
+    third = HostScene.preset(abi.PRESET_LARGE, cache_dir=tmp_path, **kw)
+    assert np.array_equal(third.geometry, plain.geometry) and np.array_equal(third.mesh_infos, plain.mesh_infos)
+    other = abi.VERTEX_FORMAT_QUANTIZED if vertex_format == abi.VERTEX_FORMAT_FP32 else abi.VERTEX_FORMAT_FP32
+    HostScene.preset(abi.PRESET_LARGE, cache_dir=tmp_path, **{**kw, "vertex_format": other})
+    assert len(list(tmp_path.glob("lhmesh-*.bin"))) == 2 * len(files)
