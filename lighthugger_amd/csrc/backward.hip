@@ -202,12 +202,17 @@ struct LightGrad {
 // gradient-arena word of the vec3 (positions and normals alike) and flushed
 // with one atomic per distinct word: neighbouring pixels share vertices, and
 // same-address atomics from one wave serialize.  A tile inserts at most
-// 64 pixels x 6 vec3 = 384 keys, so the 512-slot table never fills.
-constexpr uint32_t kHashSlots = 512;
+// 64 pixels x 6 vec3 = 384 keys: the 1024-slot table stays below 38% load
+// (short probe chains) while two blocks per SIMD still fit in LDS, and the
+// slots taken are listed so the flush visits only those.
+constexpr uint32_t kHashSlots = 1024;
+constexpr uint32_t kHashKeysMax = 384;
 constexpr uint32_t kHashEmpty = 0xFFFFFFFFu;
 struct VertexHash {
-    uint32_t* keys;  // [kHashSlots]
-    float* vals;     // [kHashSlots * 3]
+    uint32_t* keys;   // [kHashSlots]
+    float* vals;      // [kHashSlots * 3]
+    uint32_t* used;   // [kHashKeysMax] slots taken since the last flush
+    uint32_t* count;  // number of them
 };
 
 LH_DEV void add_geometry(const lh_resolve_grads& gr, const VertexHash& h, uint64_t addr, f3 g) {
@@ -215,10 +220,15 @@ LH_DEV void add_geometry(const lh_resolve_grads& gr, const VertexHash& h, uint64
     const uint64_t k64 = (addr - gr.geometry_base) / 4;
     if (k64 + 3 > gr.geometry_bytes / 4 || k64 >= kHashEmpty) return;
     const uint32_t k = uint32_t(k64);
-    uint32_t slot = (k * 2654435761u) >> 23;  // 9 bits: kHashSlots == 512
+    uint32_t slot = (k * 2654435761u) >> 22;  // 10 bits: kHashSlots == 1024
     while (true) {
         const uint32_t old = atomicCAS(h.keys + slot, kHashEmpty, k);
-        if (old == kHashEmpty || old == k) break;
+        if (old == kHashEmpty) {
+            const uint32_t pos = atomicAdd(h.count, 1u);
+            if (pos < kHashKeysMax) h.used[pos] = slot;
+            break;
+        }
+        if (old == k) break;
         slot = (slot + 1) & (kHashSlots - 1);
     }
     atomicAdd(h.vals + 3 * slot, g.x);
@@ -230,15 +240,17 @@ LH_DEV void add_geometry(const lh_resolve_grads& gr, const VertexHash& h, uint64
 // and empties the table.
 LH_DEV void flush_geometry(const lh_resolve_grads& gr, const VertexHash& h) {
     __syncthreads();
-    for (uint32_t i = threadIdx.x; i < kHashSlots; i += 64) {
-        const uint32_t k = h.keys[i];
-        if (k == kHashEmpty) continue;
-        atomicAdd(gr.geometry + k, h.vals[3 * i]);
-        atomicAdd(gr.geometry + k + 1, h.vals[3 * i + 1]);
-        atomicAdd(gr.geometry + k + 2, h.vals[3 * i + 2]);
-        h.keys[i] = kHashEmpty;
-        h.vals[3 * i] = h.vals[3 * i + 1] = h.vals[3 * i + 2] = 0.0f;
+    const uint32_t n = min(*h.count, kHashKeysMax);
+    for (uint32_t i = threadIdx.x; i < n; i += 64) {
+        const uint32_t slot = h.used[i], k = h.keys[slot];
+        atomicAdd(gr.geometry + k, h.vals[3 * slot]);
+        atomicAdd(gr.geometry + k + 1, h.vals[3 * slot + 1]);
+        atomicAdd(gr.geometry + k + 2, h.vals[3 * slot + 2]);
+        h.keys[slot] = kHashEmpty;
+        h.vals[3 * slot] = h.vals[3 * slot + 1] = h.vals[3 * slot + 2] = 0.0f;
     }
+    __syncthreads();
+    if (threadIdx.x == 0) *h.count = 0;
     __syncthreads();
 }
 
@@ -459,11 +471,14 @@ __global__ void __launch_bounds__(64) LH_BACKWARD_OCCUPANCY k_resolve_backward(l
                                                          double* partials) {
     __shared__ uint32_t hkeys[kHashSlots];
     __shared__ float hvals[kHashSlots * 3];
-    const VertexHash hash{hkeys, hvals};
+    __shared__ uint32_t hused[kHashKeysMax];
+    __shared__ uint32_t hcount;
+    const VertexHash hash{hkeys, hvals, hused, &hcount};
     for (uint32_t i = threadIdx.x; i < kHashSlots; i += 64) {
         hkeys[i] = kHashEmpty;
         hvals[3 * i] = hvals[3 * i + 1] = hvals[3 * i + 2] = 0.0f;
     }
+    if (threadIdx.x == 0) hcount = 0;
     __syncthreads();
     const lh_view& v = vs.v;
     const uint32_t tiles_x = (v.width + 7) / 8, tiles = tiles_x * ((v.height + 7) / 8);
