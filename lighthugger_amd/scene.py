@@ -40,6 +40,10 @@ def scene_lib() -> C.CDLL:
         lib.lh_scene_add_instance.argtypes = [C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p]
         lib.lh_scene_set_material_alpha_clip.restype = C.c_int32
         lib.lh_scene_set_material_alpha_clip.argtypes = [C.c_void_p, C.c_uint32, C.c_uint32]
+        lib.lh_scene_set_texture_format.restype = C.c_int32
+        lib.lh_scene_set_texture_format.argtypes = [C.c_void_p, C.c_uint32]
+        lib.lh_scene_set_cache_dir.restype = C.c_int32
+        lib.lh_scene_set_cache_dir.argtypes = [C.c_void_p, C.c_char_p]
         lib.lh_scene_set_vertex_format.restype = C.c_int32
         lib.lh_scene_set_vertex_format.argtypes = [C.c_void_p, C.c_uint32]
         lib.lh_scene_finish.restype = C.c_int32
@@ -164,15 +168,21 @@ class HostScene:
         return cls(scene_lib().lh_scene_build_preset(C.byref(cfg)))
 
     @classmethod
-    def build(cls, meshes, materials, instances, textures=(), vertex_format=abi.VERTEX_FORMAT_FP32) -> "HostScene":
+    def build(cls, meshes, materials, instances, textures=(), vertex_format=abi.VERTEX_FORMAT_FP32,
+              texture_format=abi.TEXTURE_RGBA8, cache_dir=None) -> "HostScene":
         """meshes: [(positions[N,3], normals[N,3] | None, uvs[N,2] | None, indices[T,3])];
         materials: [Material]; instances: [(mesh, material, transform[4,4] column-major)];
-        textures: [uint8 RGBA array (H, W, 4)]; vertex_format: abi.VERTEX_FORMAT_*."""
+        textures: [uint8 RGBA array (H, W, 4)]; vertex_format: abi.VERTEX_FORMAT_*;
+        texture_format: abi.TEXTURE_*; cache_dir: directory of the built-mesh cache or None."""
         lib = scene_lib()
         h = lib.lh_scene_create()
         try:
             if lib.lh_scene_set_vertex_format(h, vertex_format) != 0:
                 raise ValueError(f"unknown vertex format {vertex_format}")
+            if lib.lh_scene_set_texture_format(h, texture_format) != 0:
+                raise ValueError(f"unknown texture format {texture_format}")
+            if cache_dir is not None and lib.lh_scene_set_cache_dir(h, str(cache_dir).encode()) != 0:
+                raise ValueError(f"cache directory {cache_dir} refused")
             for pos, nrm, uv, idx in meshes:
                 pos = np.ascontiguousarray(pos, np.float32)
                 idx = np.ascontiguousarray(idx, np.uint32)
