@@ -49,7 +49,7 @@ typedef struct lh_preset_config {
     uint32_t texture_size;    /* 0 = preset default (1024 for LARGE/STRESS) */
     uint32_t mesh_tri_scale_pct; /* 0 = 100; scales per-mesh triangle targets (CPU-sized tests) */
     uint32_t vertex_format;   /* LH_VERTEX_FORMAT_* (0 = FP32) */
-    uint32_t texture_format;  /* LH_TEXTURE_RGBA8 or LH_TEXTURE_BC1 (shared_cpu_gpu.h) */
+    uint32_t texture_format;  /* LH_TEXTURE_RGBA8 / BC1 / BC7, optionally | LH_TEXTURE_SRGB (shared_cpu_gpu.h) */
     const char* cache_dir;    /* NULL, or an existing directory for the built-mesh cache */
 } lh_preset_config;
 
@@ -94,9 +94,12 @@ lh_scene_host* lh_scene_create(void);
  * vertex format, and stored there when built.  `dir` must exist; NULL or ""
  * switches the cache off.  Only before the first mesh.  0 on success. */
 int32_t lh_scene_set_cache_dir(lh_scene_host* s, const char* dir);
-/* Texture storage of the finished scene: LH_TEXTURE_RGBA8 (default) or
- * LH_TEXTURE_BC1 (each mip level encoded to BC1 blocks, decoded on fetch by
- * the kernels; alpha dropped).  Before lh_scene_finish.  0 on success. */
+/* Texture storage of the finished scene for textures added as RGBA8:
+ * LH_TEXTURE_RGBA8 (default), LH_TEXTURE_BC1 (each mip level encoded to BC1
+ * blocks, alpha dropped) or LH_TEXTURE_BC7 (mode-6 blocks, alpha kept),
+ * decoded on fetch by the kernels; | LH_TEXTURE_SRGB marks the data
+ * sRGB-encoded (decoded to linear per texel before filtering).  Encoded
+ * textures keep their own format.  Before lh_scene_finish.  0 on success. */
 int32_t lh_scene_set_texture_format(lh_scene_host* s, uint32_t format);
 /* Selects the vertex format (LH_VERTEX_FORMAT_*) of the meshes added after
  * it; only before the first mesh.  0 on success. */
@@ -109,6 +112,13 @@ int32_t lh_scene_add_mesh(lh_scene_host* s, const float* positions, const float*
 /* Adds an RGBA8 texture (level 0 only; the mip chain is built by 2x2 box
  * filtering).  Returns the texture index or -1. */
 int32_t lh_scene_add_texture(lh_scene_host* s, const uint8_t* rgba, uint32_t width, uint32_t height);
+/* Adds a texture as stored in a file (DDS, KTX2): `levels` mip levels of
+ * `format` (LH_TEXTURE_RGBA8 / BC1 / BC7, optionally | LH_TEXTURE_SRGB) packed
+ * level after level, level l of size max(1, width >> l) x max(1, height >> l)
+ * (block formats: ceil(w/4) x ceil(h/4) blocks of 8 / 16 bytes).  `bytes` must
+ * equal the sum of the level sizes.  Returns the texture index or -1. */
+int32_t lh_scene_add_texture_encoded(lh_scene_host* s, uint32_t format, uint32_t width, uint32_t height,
+                                     uint32_t levels, const uint8_t* data, uint64_t bytes);
 /* Material = (base colour factor, roughness/metallic factors, texture indices
  * or UNUSED_TEXTURE_INDEX).  Returns the material id or -1. */
 int32_t lh_scene_add_material(lh_scene_host* s, LhVec3 base_color_factor, float roughness_factor,

@@ -176,14 +176,25 @@ typedef struct LhMaterialFactors {
     float metallic_factor;
 } LhMaterialFactors;
 
-/* One texture with a full mip chain packed level after level, RGBA8 UNORM or
- * BC1 blocks (decoded on fetch).
+/* One texture with a full mip chain packed level after level, RGBA8 texels
+ * or BC1 / BC7 blocks (decoded on fetch).  format = storage | flags:
  *   LH_TEXTURE_RGBA8: 4 bytes per texel, row-major; level_offset in texels.
  *   LH_TEXTURE_BC1:   8-byte blocks of 4x4 texels, ceil(w/4) blocks per row;
- *                     level_offset in blocks. */
+ *                     level_offset in blocks.
+ *   LH_TEXTURE_BC7:   16-byte blocks of 4x4 texels, laid out as BC1.
+ *   LH_TEXTURE_SRGB:  the RGB channels are sRGB-encoded and decoded to linear
+ *                     per texel before filtering, as Vulkan samples the
+ *                     *_SRGB formats the reference uploads
+ *                     (src/resources/image_loading.cpp:14-39); alpha is linear.
+ *   LH_TEXTURE_RGB:   alpha reads as 1 (Vulkan's BC1_RGB formats, where a
+ *                     three-colour block's fourth entry is opaque black). */
 #define LH_MAX_TEXTURE_LEVELS 16u
 #define LH_TEXTURE_RGBA8 0u
 #define LH_TEXTURE_BC1 1u
+#define LH_TEXTURE_BC7 2u
+#define LH_TEXTURE_FORMAT_MASK 0xFFu
+#define LH_TEXTURE_SRGB 0x100u
+#define LH_TEXTURE_RGB 0x200u
 typedef struct LhTexture {
     uint64_t texels; /* device address of level 0 */
     uint32_t width;

@@ -14,7 +14,8 @@ MESH_INFO_FLAGS_ALPHA_CLIP = 0x02
 MESH_INFO_FLAGS_FP32_VERTICES = 0x04
 SCENE_OPAQUE_ONLY = 0x1  # LH_SCENE_OPAQUE_ONLY
 TONEMAP_LUT_DIMS = 48  # LH_TONEMAP_LUT_DIMS
-TEXTURE_RGBA8, TEXTURE_BC1 = 0, 1  # LhTexture.format
+TEXTURE_RGBA8, TEXTURE_BC1, TEXTURE_BC7 = 0, 1, 2  # LhTexture.format storage
+TEXTURE_FORMAT_MASK, TEXTURE_SRGB, TEXTURE_RGB = 0xFF, 0x100, 0x200  # flags: sRGB-encoded RGB, alpha read as 1
 # Uniforms.debug (src/shared_cpu_gpu.h:46-51)
 (UNIFORMS_DEBUG_OFF, UNIFORMS_DEBUG_CASCADES, UNIFORMS_DEBUG_TRIANGLE_INDEX, UNIFORMS_DEBUG_INSTANCE_INDEX,
  UNIFORMS_DEBUG_SHADER_CLOCK, UNIFORMS_DEBUG_NORMALS) = range(6)

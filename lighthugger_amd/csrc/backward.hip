@@ -156,9 +156,7 @@ LH_DEV f4 texel(const LhTexture& t, uint32_t level, int32_t w, int32_t h, int32_
     if (i < 0) i += w;
     j %= h;
     if (j < 0) j += h;
-    const uint32_t q = texel_rgba8(t, level, uint32_t(w), uint32_t(i), uint32_t(j));
-    return {float(q & 255u) / 255.0f, float((q >> 8) & 255u) / 255.0f, float((q >> 16) & 255u) / 255.0f,
-            float(q >> 24) / 255.0f};
+    return unpack_texel(t, texel_rgba8(t, level, uint32_t(w), uint32_t(i), uint32_t(j)));
 }
 // Adds weight times the bilinear sample at one level (and its uv derivatives).
 // The texel cell and its fractions come from FP64 coordinates: FP32 would
@@ -310,14 +308,19 @@ LH_DEV void pixel_bwd(const lh_scene& s, const lh_view& v, const lh_resolve_para
         return;
     }
     Frozen fz;
-    if constexpr (kRecorded) {  // recorded by the forward (lh_resolve_ex with the same buffer)
-        const uint4 r = reinterpret_cast<const uint4*>(prm.frozen)[o];
+    // The record of the forward (lh_resolve_ex with the same buffer) is used
+    // only where its valid bit is set; its levels are clamped to the
+    // texture's chain, so a stale or foreign record cannot address texels
+    // outside the texture.  Elsewhere the quantities are recomputed.
+    const uint4 r = kRecorded ? reinterpret_cast<const uint4*>(prm.frozen)[o] : make_uint4(0, 0, 0, 0);
+    if (kRecorded && (r.w >> 31) != 0u) {
         fz.flip = (r.w & 1u) ? -1.0f : 1.0f;
         fz.K = kPi * __uint_as_float(r.x);
         auto lod = [&](uint32_t index, uint32_t l0, uint32_t fbits) {
             if (!has_texture(s, index)) return Lod{0, 0, 0.0f};
             const uint32_t top = s.textures[index].levels - 1;
-            return Lod{l0, l0 + 1 < top ? l0 + 1 : top, __uint_as_float(fbits)};
+            const uint32_t a = min(l0, top);
+            return Lod{a, a + 1 < top ? a + 1 : top, clampf(__uint_as_float(fbits), 0.0f, 1.0f)};
         };
         fz.lod_b = lod(mi.base_color_texture_index, (r.w >> 8) & 255u, r.y);
         fz.lod_m = lod(mi.metallic_roughness_texture_index, (r.w >> 16) & 255u, r.z);

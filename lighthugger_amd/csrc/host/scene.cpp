@@ -550,8 +550,10 @@ void build_meshlets(BuiltMesh& bm) {
 
 struct TextureData {
     uint32_t w = 0, h = 0, levels = 0;
-    std::vector<uint32_t> level_offset;  // texels
-    std::vector<uint8_t> rgba;           // all levels
+    std::vector<uint32_t> level_offset;  // texels (or blocks of an encoded texture)
+    std::vector<uint8_t> rgba;           // all levels (the stored data of an encoded texture)
+    bool encoded = false;                // stored as given: `format`, `rgba` holds the levels
+    uint32_t format = LH_TEXTURE_RGBA8;
 };
 
 // BC1 (S3TC) encoder of the producer.  Per 4x4 block the endpoints are the
@@ -614,6 +616,108 @@ void encode_bc1_level(const uint8_t* rgba, uint32_t w, uint32_t h, std::vector<u
             const uint8_t blk[8] = {uint8_t(c0), uint8_t(c0 >> 8), uint8_t(c1), uint8_t(c1 >> 8),
                                     uint8_t(idx), uint8_t(idx >> 8), uint8_t(idx >> 16), uint8_t(idx >> 24)};
             out.insert(out.end(), blk, blk + 8);
+        }
+}
+
+// BC7 encoder of the producer: mode 6 only (one subset; RGBA endpoints of 7
+// bits plus one p-bit per endpoint, 4-bit indices).  Per 4x4 block the
+// endpoints are the extremes of the texels projected on the principal axis
+// of their RGBA covariance (power iteration), each channel rounded to the
+// nearest value whose lowest bit is the endpoint's p-bit (the majority of its
+// channels' lowest bits); every texel takes the index whose interpolated RGBA,
+// as the decoder forms it, is nearest in squared distance.  Texel 0 is the
+// anchor and stores 3 index bits, so when its index exceeds 7 the endpoints
+// swap and every index mirrors to 15 - i (the weights are symmetric).  Texels
+// beyond the level's edge repeat the edge.
+struct Bits128 {
+    uint64_t w[2] = {0, 0};
+    uint32_t pos = 0;
+    void put(uint32_t v, uint32_t n) {
+        for (uint32_t k = 0; k < n; k++, pos++)
+            if ((v >> k) & 1u) w[pos >> 6] |= 1ull << (pos & 63);
+    }
+};
+void encode_bc7_level(const uint8_t* rgba, uint32_t w, uint32_t h, std::vector<uint8_t>& out) {
+    static const int kW4[16] = {0, 4, 9, 13, 17, 21, 26, 30, 34, 38, 43, 47, 51, 55, 60, 64};
+    for (uint32_t by = 0; by < (h + 3) / 4; by++)
+        for (uint32_t bx = 0; bx < (w + 3) / 4; bx++) {
+            const uint8_t* px[16];
+            for (uint32_t k = 0; k < 16; k++) {
+                uint32_t x = std::min(w - 1, bx * 4 + (k & 3)), y = std::min(h - 1, by * 4 + (k >> 2));
+                px[k] = rgba + 4 * (size_t(y) * w + x);
+            }
+            double mean[4] = {0, 0, 0, 0}, cov[4][4] = {};
+            for (int k = 0; k < 16; k++)
+                for (int c = 0; c < 4; c++) mean[c] += px[k][c] / 16.0;
+            for (int k = 0; k < 16; k++)
+                for (int a = 0; a < 4; a++)
+                    for (int b = 0; b < 4; b++) cov[a][b] += (px[k][a] - mean[a]) * (px[k][b] - mean[b]);
+            double axis[4] = {0, 0, 0, 0};
+            int widest = 0;
+            for (int c = 1; c < 4; c++)
+                if (cov[c][c] > cov[widest][widest]) widest = c;
+            axis[widest] = 1.0;
+            for (int it = 0; it < 16; it++) {
+                double nx[4] = {0, 0, 0, 0}, len = 0;
+                for (int a = 0; a < 4; a++) {
+                    for (int b = 0; b < 4; b++) nx[a] += cov[a][b] * axis[b];
+                    len += nx[a] * nx[a];
+                }
+                if (len <= 1e-12) break;
+                for (int a = 0; a < 4; a++) axis[a] = nx[a] / std::sqrt(len);
+            }
+            double tmin = 0, tmax = 0;
+            for (int k = 0; k < 16; k++) {
+                double t = 0;
+                for (int c = 0; c < 4; c++) t += (px[k][c] - mean[c]) * axis[c];
+                tmin = std::min(tmin, t);
+                tmax = std::max(tmax, t);
+            }
+            int ep[2][4];
+            for (int c = 0; c < 4; c++) {
+                ep[0][c] = int(std::lround(std::clamp(mean[c] + tmin * axis[c], 0.0, 255.0)));
+                ep[1][c] = int(std::lround(std::clamp(mean[c] + tmax * axis[c], 0.0, 255.0)));
+            }
+            uint32_t pbit[2];
+            for (int e = 0; e < 2; e++) {
+                int odd = 0;
+                for (int c = 0; c < 4; c++) odd += ep[e][c] & 1;
+                pbit[e] = odd >= 2 ? 1u : 0u;
+                for (int c = 0; c < 4; c++)
+                    if (uint32_t(ep[e][c] & 1) != pbit[e]) ep[e][c] += ep[e][c] == 255 ? -1 : 1;
+            }
+            int pal[16][4];
+            for (int i = 0; i < 16; i++)
+                for (int c = 0; c < 4; c++) pal[i][c] = ((64 - kW4[i]) * ep[0][c] + kW4[i] * ep[1][c] + 32) >> 6;
+            uint32_t idx[16];
+            for (int k = 0; k < 16; k++) {
+                int best = 0, bd = 1 << 30;
+                for (int i = 0; i < 16; i++) {
+                    int d = 0;
+                    for (int c = 0; c < 4; c++) d += (int(px[k][c]) - pal[i][c]) * (int(px[k][c]) - pal[i][c]);
+                    if (d < bd) {
+                        bd = d;
+                        best = i;
+                    }
+                }
+                idx[k] = uint32_t(best);
+            }
+            if (idx[0] >= 8) {
+                for (int c = 0; c < 4; c++) std::swap(ep[0][c], ep[1][c]);
+                std::swap(pbit[0], pbit[1]);
+                for (int k = 0; k < 16; k++) idx[k] = 15u - idx[k];
+            }
+            Bits128 b;
+            b.put(1u << 6, 7);  // mode 6
+            for (int c = 0; c < 4; c++)
+                for (int e = 0; e < 2; e++) b.put(uint32_t(ep[e][c]) >> 1, 7);
+            b.put(pbit[0], 1);
+            b.put(pbit[1], 1);
+            b.put(idx[0], 3);
+            for (int k = 1; k < 16; k++) b.put(idx[k], 4);
+            uint8_t blk[16];
+            std::memcpy(blk, b.w, 16);
+            out.insert(out.end(), blk, blk + 16);
         }
 }
 
@@ -820,12 +924,16 @@ uint64_t append_bytes(std::vector<uint8_t>& arena, const std::vector<T>& v) {
 
 // ---- on-disk cache of built meshes (the reference's FsCache,
 // src/resources/fs_cache.h:3-25, which stores built meshlets per asset).
-// Key: FNV-1a 64 of the source arrays, the vertex format and the file
-// version; file: <dir>/lhmesh-<key>.bin holding the built mesh (reordered
-// vertices, meshlets, remap, micro indices, bounds, quantization grid).  A
-// file whose checksum or sizes do not hold is ignored (the mesh is rebuilt).
+// Key: FNV-1a 64 of the source arrays, the vertex format, the file version
+// and the builder's identity (its limits and revision); file:
+// <dir>/lhmesh-<key>.bin holding the built mesh (reordered vertices,
+// meshlets, remap, micro indices, bounds, quantization grid).  A file whose
+// checksum, sizes or meshlet limits do not hold is ignored (the mesh is
+// rebuilt).  kBuilderRevision changes with the meshlet builder's output.
 constexpr uint32_t kCacheMagic = 0x434D484Cu;  // "LHMC"
 constexpr uint32_t kCacheVersion = 1;
+constexpr uint32_t kBuilderRevision = 1;
+constexpr uint32_t kBuilderIdentity[3] = {kBuilderRevision, MAX_MESHLET_UNIQUE_VERTICES, MAX_MESHLET_TRIANGLES};
 
 uint64_t fnv1a(const void* data, size_t n, uint64_t h = 1469598103934665603ull) {
     const uint8_t* p = static_cast<const uint8_t*>(data);
@@ -840,6 +948,7 @@ uint64_t fnv1a_vec(const std::vector<T>& v, uint64_t h) {
 }
 std::string cache_path(const lh_scene_host* s, const MeshSource& m) {
     uint64_t h = fnv1a(&kCacheVersion, sizeof kCacheVersion);
+    h = fnv1a(kBuilderIdentity, sizeof kBuilderIdentity, h);
     h = fnv1a(&s->vertex_format, sizeof s->vertex_format, h);
     h = fnv1a_vec(m.pos, h);
     h = fnv1a_vec(m.nrm, h);
@@ -932,11 +1041,15 @@ bool load_cached(const std::string& path, BuiltMesh& bm, const MeshSource& src) 
               r.at == buf.size();
     ok = ok && c.src.pos.size() == 3 * nv && c.src.nrm.size() == 3 * nv && c.src.uv.size() == 2 * nv &&
          c.src.idx.size() == 3 * nt && !c.meshlets.empty();
-    // every meshlet's ranges within the stored arrays, every remap entry a vertex
+    // every meshlet within the limits the raster assumes (<= 64 vertices,
+    // <= 124 triangles), its ranges within the stored arrays, every micro
+    // index below its vertex count, every remap entry a vertex
     for (size_t k = 0; ok && k < c.meshlets.size(); k++) {
         const Meshlet& ml = c.meshlets[k];
-        ok = size_t(ml.index_offset) + ml.index_count <= c.remap.size() &&
-             size_t(ml.triangle_offset) + 3u * ml.triangle_count <= c.micro.size() && ml.triangle_count > 0;
+        ok = ml.index_count > 0 && ml.index_count <= MAX_MESHLET_UNIQUE_VERTICES && ml.triangle_count > 0 &&
+             ml.triangle_count <= MAX_MESHLET_TRIANGLES && size_t(ml.index_offset) + ml.index_count <= c.remap.size() &&
+             size_t(ml.triangle_offset) + 3u * ml.triangle_count <= c.micro.size();
+        for (uint32_t t = 0; ok && t < 3u * ml.triangle_count; t++) ok = c.micro[ml.triangle_offset + t] < ml.index_count;
     }
     for (size_t k = 0; ok && k < c.remap.size(); k++) ok = c.remap[k] < nv;
     if (!ok) return false;
@@ -973,7 +1086,9 @@ int32_t lh_scene_set_cache_dir(lh_scene_host* s, const char* dir) {
 }
 
 int32_t lh_scene_set_texture_format(lh_scene_host* s, uint32_t format) {
-    if (!s || s->finished || format > LH_TEXTURE_BC1) return -1;
+    if (!s || s->finished || (format & ~(LH_TEXTURE_FORMAT_MASK | LH_TEXTURE_SRGB)) ||
+        (format & LH_TEXTURE_FORMAT_MASK) > LH_TEXTURE_BC7)
+        return -1;
     s->texture_format = format;
     return 0;
 }
@@ -1008,6 +1123,34 @@ int32_t lh_scene_add_texture(lh_scene_host* s, const uint8_t* rgba, uint32_t wid
     t.h = height;
     t.rgba.assign(rgba, rgba + size_t(width) * height * 4);
     build_mips(t);
+    s->textures.push_back(std::move(t));
+    return int32_t(s->textures.size() - 1);
+}
+
+int32_t lh_scene_add_texture_encoded(lh_scene_host* s, uint32_t format, uint32_t width, uint32_t height,
+                                     uint32_t levels, const uint8_t* data, uint64_t bytes) {
+    if (!s || s->finished || !data || width == 0 || height == 0 || levels == 0 || levels > LH_MAX_TEXTURE_LEVELS ||
+        (format & ~(LH_TEXTURE_FORMAT_MASK | LH_TEXTURE_SRGB)) || (format & LH_TEXTURE_FORMAT_MASK) > LH_TEXTURE_BC7)
+        return -1;
+    // A chain may stop early, but not beyond 1 x 1.
+    if (levels > 1 && (std::max(width, height) >> (levels - 1)) == 0) return -1;
+    const uint32_t fmt = format & LH_TEXTURE_FORMAT_MASK;
+    const uint64_t unit = fmt == LH_TEXTURE_BC1 ? 8 : fmt == LH_TEXTURE_BC7 ? 16 : 4;
+    TextureData t;
+    t.w = width;
+    t.h = height;
+    t.levels = levels;
+    t.encoded = true;
+    t.format = format;
+    uint64_t n = 0;
+    for (uint32_t l = 0; l < levels; l++) {
+        const uint64_t w = std::max(1u, width >> l), h = std::max(1u, height >> l);
+        if (n > 0xFFFFFFFFull) return -1;
+        t.level_offset.push_back(uint32_t(n));
+        n += fmt == LH_TEXTURE_RGBA8 ? w * h : ((w + 3) / 4) * ((h + 3) / 4);
+    }
+    if (n * unit != bytes) return -1;
+    t.rgba.assign(data, data + bytes);
     s->textures.push_back(std::move(t));
     return int32_t(s->textures.size() - 1);
 }
@@ -1150,13 +1293,22 @@ int32_t lh_scene_finish(lh_scene_host* s) {
         lt.width = t.w;
         lt.height = t.h;
         lt.levels = t.levels;
-        lt.format = s->texture_format;
-        if (s->texture_format == LH_TEXTURE_BC1) {
+        lt.format = t.encoded ? t.format : s->texture_format;
+        const uint32_t fmt = lt.format & LH_TEXTURE_FORMAT_MASK;
+        if (t.encoded) {
+            lt.texels = append_bytes(s->texels, t.rgba);
+            for (uint32_t l = 0; l < t.levels; l++) lt.level_offset[l] = t.level_offset[l];
+        } else if (fmt == LH_TEXTURE_BC1 || fmt == LH_TEXTURE_BC7) {
+            const size_t unit = fmt == LH_TEXTURE_BC1 ? 8 : 16;
             std::vector<uint8_t> blocks;
             for (uint32_t l = 0; l < t.levels; l++) {
-                lt.level_offset[l] = uint32_t(blocks.size() / 8);
+                lt.level_offset[l] = uint32_t(blocks.size() / unit);
                 const uint32_t w = std::max(1u, t.w >> l), h = std::max(1u, t.h >> l);
-                encode_bc1_level(t.rgba.data() + size_t(t.level_offset[l]) * 4, w, h, blocks);
+                const uint8_t* src = t.rgba.data() + size_t(t.level_offset[l]) * 4;
+                if (fmt == LH_TEXTURE_BC1)
+                    encode_bc1_level(src, w, h, blocks);
+                else
+                    encode_bc7_level(src, w, h, blocks);
             }
             lt.texels = append_bytes(s->texels, blocks);
         } else {

@@ -126,11 +126,170 @@ LH_DEV uint32_t bc1_colour(uint32_t c0, uint32_t c1, uint32_t sel) {
     }
     return out;
 }
+// BC7 (BPTC) texel `t` (row order, x fastest) of one 16-byte block, packed
+// RGBA8.  Random access into the block: the mode (unary prefix of the first
+// byte) fixes the field widths, so the texel's subset, its two endpoints,
+// their p-bits and its index(es) are read at computed bit offsets without
+// decoding the rest of the block.  Endpoints with a p-bit append it and are
+// widened to 8 bits by replicating their top bits; the texel is
+// ((64 - w) e0 + w e1 + 32) >> 6 with the 2-, 3- or 4-bit weights; rotation
+// swaps alpha with R, G or B.  A block without a mode bit in its first byte
+// is transparent black.
+namespace bc7 {
+// per mode: subsets, partition bits, rotation bits, index-selection bits,
+// colour bits, alpha bits, per-endpoint p-bits, shared p-bits, index bits,
+// second index bits - packed 4 bits each, lowest first
+static __constant__ uint64_t kModes[8] = {
+    0x0000000301040043ull, 0x0000000310060062ull, 0x0000000200050063ull, 0x0000000201070062ull,
+    0x0000003200651201ull, 0x0000002200870201ull, 0x0000000401770001ull, 0x0000000201550062ull,
+};
+// two-subset partitions: bit t of entry p = subset of texel t
+static __constant__ uint16_t kPart2[64] = {
+    0xCCCC, 0x8888, 0xEEEE, 0xECC8, 0xC880, 0xFEEC, 0xFEC8, 0xEC80, 0xC800, 0xFFEC, 0xFE80, 0xE800, 0xFFE8,
+    0xFF00, 0xFFF0, 0xF000, 0xF710, 0x008E, 0x7100, 0x08CE, 0x008C, 0x7310, 0x3100, 0x8CCE, 0x088C, 0x3110,
+    0x6666, 0x366C, 0x17E8, 0x0FF0, 0x718E, 0x399C, 0xAAAA, 0xF0F0, 0x5A5A, 0x33CC, 0x3C3C, 0x55AA, 0x9696,
+    0xA55A, 0x73CE, 0x13C8, 0x324C, 0x3BDC, 0x6996, 0xC33C, 0x9966, 0x0660, 0x0272, 0x04E4, 0x4E40, 0x2720,
+    0xC936, 0x936C, 0x39C6, 0x639C, 0x9336, 0x9CC6, 0x817E, 0xE718, 0xCCF0, 0x0FCC, 0x7744, 0xEE22,
+};
+// three-subset partitions: bits 2t..2t+1 of entry p = subset of texel t
+static __constant__ uint32_t kPart3[64] = {
+    0xAA685050, 0x6A5A5040, 0x5A5A4200, 0x5450A0A8, 0xA5A50000, 0xA0A05050, 0x5555A0A0, 0x5A5A5050,
+    0xAA550000, 0xAA555500, 0xAAAA5500, 0x90909090, 0x94949494, 0xA4A4A4A4, 0xA9A59450, 0x2A0A4250,
+    0xA5945040, 0x0A425054, 0xA5A5A500, 0x55A0A0A0, 0xA8A85454, 0x6A6A4040, 0xA4A45000, 0x1A1A0500,
+    0x0050A4A4, 0xAAA59090, 0x14696914, 0x69691400, 0xA08585A0, 0xAA821414, 0x50A4A450, 0x6A5A0200,
+    0xA9A58000, 0x5090A0A8, 0xA8A09050, 0x24242424, 0x00AA5500, 0x24924924, 0x24499224, 0x50A50A50,
+    0x500AA550, 0xAAAA4444, 0x66660000, 0xA5A0A5A0, 0x50A050A0, 0x69286928, 0x44AAAA44, 0x66666600,
+    0xAA444444, 0x54A854A8, 0x95809580, 0x96969600, 0xA85454A8, 0x80959580, 0xAA141414, 0x96960000,
+    0xAAAA1414, 0xA05050A0, 0xA0A5A5A0, 0x96000000, 0x40804080, 0xA9A8A9A8, 0xAAAAAA44, 0x2A4A5254,
+};
+// anchor texels: subset 1 of two; subsets 1 and 2 of three
+static __constant__ uint8_t kAnchor2[64] = {15, 15, 15, 15, 15, 15, 15, 15, 15, 15, 15, 15, 15, 15, 15, 15,
+                                            15, 2,  8,  2,  2,  8,  8,  15, 2,  8,  2,  2,  8,  8,  2,  2,
+                                            15, 15, 6,  8,  2,  8,  15, 15, 2,  8,  2,  2,  2,  15, 15, 6,
+                                            6,  2,  6,  8,  15, 15, 2,  2,  15, 15, 15, 15, 15, 2,  2,  15};
+static __constant__ uint8_t kAnchor3a[64] = {3,  3,  15, 15, 8,  3,  15, 15, 8,  8,  6,  6,  6,  5,  3,  3,
+                                             3,  3,  8,  15, 3,  3,  6,  10, 5,  8,  8,  6,  8,  5,  15, 15,
+                                             8,  15, 3,  5,  6,  10, 8,  15, 15, 3,  15, 5,  15, 15, 15, 15,
+                                             3,  15, 5,  5,  5,  8,  5,  10, 5,  10, 8,  13, 15, 12, 3,  3};
+static __constant__ uint8_t kAnchor3b[64] = {15, 8,  8,  3,  15, 15, 3,  8,  15, 15, 15, 15, 15, 15, 15, 8,
+                                             15, 8,  15, 3,  15, 8,  15, 8,  3,  15, 6,  10, 15, 15, 10, 8,
+                                             15, 3,  15, 10, 10, 8,  9,  10, 6,  15, 8,  15, 3,  6,  6,  8,
+                                             15, 3,  15, 15, 15, 15, 15, 15, 15, 15, 15, 15, 3,  15, 15, 8};
+// interpolation weights of 2-, 3- and 4-bit indices: kWeights[offset + index], offsets 0, 4, 12
+static __constant__ uint8_t kWeights[28] = {0,  21, 43, 64, 0,  9,  18, 27, 37, 46, 55, 64, 0,  4,
+                                            9,  13, 17, 21, 26, 30, 34, 38, 43, 47, 51, 55, 60, 64};
+
+struct Block {
+    uint64_t lo, hi;
+    // n <= 32 bits at bit position pos (< 128)
+    LH_DEV uint32_t bits(uint32_t pos, uint32_t n) const {
+        const uint64_t v = pos >= 64 ? hi >> (pos - 64) : pos == 0 ? lo : (lo >> pos) | (hi << (64 - pos));
+        return uint32_t(v) & uint32_t((1ull << n) - 1);
+    }
+};
+LH_DEV uint32_t widen(uint32_t v, uint32_t bits) { return (v << (8 - bits)) | (v >> (2 * bits - 8)); }
+LH_DEV uint32_t interp(uint32_t a, uint32_t b, uint32_t w) { return ((64 - w) * a + w * b + 32) >> 6; }
+LH_DEV uint32_t weight(uint32_t bits, uint32_t i) { return kWeights[(bits == 2 ? 0u : bits == 3 ? 4u : 12u) + i]; }
+
+LH_DEV uint32_t texel(uint4 q, uint32_t t) {
+    const uint32_t first = q.x & 255u;
+    if (first == 0) return 0u;
+    const uint32_t mode = uint32_t(__builtin_ctz(first));
+    const uint64_t mi = kModes[mode];
+    auto f = [&](int k) { return uint32_t(mi >> (4 * k)) & 15u; };
+    const uint32_t ns = f(0), pb = f(1), rb = f(2), isb = f(3), cb = f(4), ab = f(5), epb = f(6), spb = f(7),
+                   ib = f(8), ib2 = f(9);
+    const Block b{uint64_t(q.x) | (uint64_t(q.y) << 32), uint64_t(q.z) | (uint64_t(q.w) << 32)};
+    uint32_t pos = mode + 1;
+    const uint32_t part = b.bits(pos, pb);
+    pos += pb;
+    const uint32_t rot = b.bits(pos, rb);
+    pos += rb;
+    const uint32_t sel = b.bits(pos, isb);
+    pos += isb;
+    const uint32_t s = ns == 1 ? 0u : ns == 2 ? (kPart2[part] >> t) & 1u : (kPart3[part] >> (2 * t)) & 3u;
+    const uint32_t ne = 2 * ns, k0 = 2 * s, k1 = 2 * s + 1;
+    uint32_t e0[4], e1[4];
+    for (uint32_t c = 0; c < 3; c++) {
+        e0[c] = b.bits(pos + (c * ne + k0) * cb, cb);
+        e1[c] = b.bits(pos + (c * ne + k1) * cb, cb);
+    }
+    pos += 3 * ne * cb;
+    e0[3] = e1[3] = 255u;
+    if (ab) {
+        e0[3] = b.bits(pos + k0 * ab, ab);
+        e1[3] = b.bits(pos + k1 * ab, ab);
+        pos += ne * ab;
+    }
+    uint32_t p0 = 0, p1 = 0, extra = 0;
+    if (epb) {
+        p0 = b.bits(pos + k0, 1);
+        p1 = b.bits(pos + k1, 1);
+        pos += ne;
+        extra = 1;
+    } else if (spb) {
+        p0 = p1 = b.bits(pos + s, 1);
+        pos += ns;
+        extra = 1;
+    }
+    for (uint32_t c = 0; c < 4; c++) {
+        if (c == 3 && !ab) continue;
+        const uint32_t n = (c < 3 ? cb : ab) + extra;
+        e0[c] = widen(extra ? (e0[c] << 1) | p0 : e0[c], n);
+        e1[c] = widen(extra ? (e1[c] << 1) | p1 : e1[c], n);
+    }
+    // primary indices: the anchor texel of each subset stores ib - 1 bits
+    const uint32_t a1 = ns == 2 ? kAnchor2[part] : ns == 3 ? kAnchor3a[part] : 16u;
+    const uint32_t a2 = ns == 3 ? kAnchor3b[part] : 16u;
+    const uint32_t off = t * ib - (t > 0 ? 1u : 0u) - (t > a1 ? 1u : 0u) - (t > a2 ? 1u : 0u);
+    const bool anchor = t == 0 || t == a1 || t == a2;
+    const uint32_t i1 = b.bits(pos + off, ib - (anchor ? 1u : 0u));
+    uint32_t ci = i1, cbits = ib, ai = i1, abits = ib;
+    if (ib2) {  // modes 4 and 5: a second index plane, anchor texel 0 only
+        const uint32_t i2 = b.bits(pos + 16 * ib - ns + t * ib2 - (t > 0 ? 1u : 0u), ib2 - (t == 0 ? 1u : 0u));
+        if (sel) {
+            ci = i2, cbits = ib2;
+        } else {
+            ai = i2, abits = ib2;
+        }
+    }
+    uint32_t px[4];
+    for (uint32_t c = 0; c < 3; c++) px[c] = interp(e0[c], e1[c], weight(cbits, ci));
+    px[3] = ab ? interp(e0[3], e1[3], weight(abits, ai)) : 255u;
+    if (rot) {
+        const uint32_t tmp = px[3];
+        px[3] = px[rot - 1];
+        px[rot - 1] = tmp;
+    }
+    return px[0] | (px[1] << 8) | (px[2] << 16) | (px[3] << 24);
+}
+}  // namespace bc7
+
 // Texel (i, j) of a level of width w (already wrapped), packed RGBA8.
 LH_DEV uint32_t texel_rgba8(const LhTexture& t, uint32_t level, uint32_t w, uint32_t i, uint32_t j) {
-    if (t.format != LH_TEXTURE_BC1) return ptr<uint32_t>(t.texels)[size_t(t.level_offset[level]) + size_t(j) * w + i];
-    const uint2 blk = ptr<uint2>(t.texels)[size_t(t.level_offset[level]) + size_t(j >> 2) * ((w + 3) >> 2) + (i >> 2)];
-    return bc1_colour(blk.x & 0xFFFFu, blk.x >> 16, (blk.y >> (2 * (4 * (j & 3) + (i & 3)))) & 3u);
+    const uint32_t fmt = t.format & LH_TEXTURE_FORMAT_MASK;
+    const size_t block = size_t(t.level_offset[level]) + size_t(j >> 2) * ((w + 3) >> 2) + (i >> 2);
+    if (fmt == LH_TEXTURE_BC1) {
+        const uint2 blk = ptr<uint2>(t.texels)[block];
+        return bc1_colour(blk.x & 0xFFFFu, blk.x >> 16, (blk.y >> (2 * (4 * (j & 3) + (i & 3)))) & 3u) |
+               ((t.format & LH_TEXTURE_RGB) ? 0xFF000000u : 0u);
+    }
+    if (fmt == LH_TEXTURE_BC7) return bc7::texel(ptr<uint4>(t.texels)[block], 4 * (j & 3) + (i & 3));
+    return ptr<uint32_t>(t.texels)[size_t(t.level_offset[level]) + size_t(j) * w + i];
+}
+
+// sRGB EOTF of an 8-bit channel: c / 12.92 up to 0.04045, else ((c + 0.055) / 1.055)^2.4.
+LH_DEV float srgb_to_linear(uint32_t v) {
+    const float c = float(v) / 255.0f;
+    return c <= 0.04045f ? c / 12.92f : powf((c + 0.055f) / 1.055f, 2.4f);
+}
+// A packed RGBA8 texel as floats in [0, 1]; sRGB formats decode RGB to linear.
+LH_DEV f4 unpack_texel(const LhTexture& t, uint32_t p) {
+    if (t.format & LH_TEXTURE_SRGB)
+        return {srgb_to_linear(p & 255u), srgb_to_linear((p >> 8) & 255u), srgb_to_linear((p >> 16) & 255u),
+                float(p >> 24) / 255.0f};
+    return {float(p & 255u) / 255.0f, float((p >> 8) & 255u) / 255.0f, float((p >> 16) & 255u) / 255.0f,
+            float(p >> 24) / 255.0f};
 }
 
 LH_DEV f4 texel(const LhTexture& t, uint32_t level, int32_t i, int32_t j) {
@@ -139,9 +298,7 @@ LH_DEV f4 texel(const LhTexture& t, uint32_t level, int32_t i, int32_t j) {
     if (i < 0) i += w;
     j %= h;
     if (j < 0) j += h;
-    const uint32_t p = texel_rgba8(t, level, uint32_t(w), uint32_t(i), uint32_t(j));
-    return {float(p & 255u) / 255.0f, float((p >> 8) & 255u) / 255.0f, float((p >> 16) & 255u) / 255.0f,
-            float(p >> 24) / 255.0f};
+    return unpack_texel(t, texel_rgba8(t, level, uint32_t(w), uint32_t(i), uint32_t(j)));
 }
 LH_DEV f4 bilinear(const LhTexture& t, uint32_t level, f2 uv) {
     const float w = float(max(1u, t.width >> level)), h = float(max(1u, t.height >> level));

@@ -1,17 +1,27 @@
 """DDS reading for the texture files the reference ships (load_dds,
-src/resources/image_loading.cpp:67-160, header layout src/resources/dds.h):
+src/resources/image_loading.cpp:67-240, header layout src/resources/dds.h):
 magic "DDS ", the 124-byte DDS_HEADER and the DDS_HEADER_DXT10 extension
 (the reference always reads it; files with the legacy "DXT1" FourCC and no
 extension are accepted too).
 
     lut = load_tonemap_lut("tony_mc_mapface.dds")   # 48^3 RGBA float for lh_view.tonemap_lut
-    img = read_dds("albedo.dds")                     # levels decoded to RGBA8 (BC1, RGBA8) or float (RGB9E5)
+    tex = load_dds_texture("albedo.dds")             # EncodedTexture for HostScene.build(textures=...)
+    img = read_dds("albedo.dds")                     # levels decoded (BC1, RGBA8, RGB9E5, RGBA32F)
+    raw = read_dds("albedo.dds", raw=True)           # levels as stored (every format, BC7 included)
 
 Formats: DXGI_FORMAT_R9G9B9E5_SHAREDEXP (67; shared-exponent floats, the
-tony-mc-mapface LUT), DXGI_FORMAT_BC1_UNORM(_SRGB) (71, 72; decoded with the
-published BC1 rules the kernels use), DXGI_FORMAT_R8G8B8A8_UNORM(_SRGB) (28,
-29), DXGI_FORMAT_R32G32B32A32_FLOAT (2).  BC7 and zstd-supercompressed KTX2
-are not read.
+tony-mc-mapface LUT and the skybox), DXGI_FORMAT_BC1_UNORM(_SRGB) (71, 72),
+DXGI_FORMAT_BC7_UNORM(_SRGB) (98, 99), DXGI_FORMAT_R8G8B8A8_UNORM(_SRGB) (28,
+29), DXGI_FORMAT_R32G32B32A32_FLOAT (2).  As the reference uploads them
+(translate_format, src/resources/image_loading.cpp:14-39), BC1_UNORM and both
+BC7 formats become sRGB textures (Vulkan's BC1_RGB_SRGB / BC7_SRGB): the
+sampler decodes them to linear before filtering, and BC1 reads alpha 1.
+
+Cube maps: the six faces are returned in the DDS order (+X, -X, +Y, -Y, +Z,
+-Z), each face with its own mip chain, face after face as DDS files store
+them.  The reference's copy regions read every level's six faces
+consecutively (src/resources/image_loading.cpp:216-240), which is the same
+order for single-level cube maps such as its skybox.
 """
 from __future__ import annotations
 
@@ -20,9 +30,22 @@ from pathlib import Path
 
 import numpy as np
 
+from . import abi
+
 R32G32B32A32_FLOAT, R8G8B8A8_UNORM, R8G8B8A8_UNORM_SRGB = 2, 28, 29
 R9G9B9E5_SHAREDEXP, BC1_UNORM, BC1_UNORM_SRGB = 67, 71, 72
+BC7_UNORM, BC7_UNORM_SRGB = 98, 99
 _DDSCAPS2_CUBEMAP = 0x200
+
+# DXGI format -> LhTexture.format, as the reference maps them to Vulkan formats
+TEXTURE_FORMAT = {
+    BC1_UNORM: abi.TEXTURE_BC1 | abi.TEXTURE_SRGB | abi.TEXTURE_RGB,
+    BC1_UNORM_SRGB: abi.TEXTURE_BC1 | abi.TEXTURE_SRGB | abi.TEXTURE_RGB,
+    BC7_UNORM: abi.TEXTURE_BC7 | abi.TEXTURE_SRGB,
+    BC7_UNORM_SRGB: abi.TEXTURE_BC7 | abi.TEXTURE_SRGB,
+    R8G8B8A8_UNORM: abi.TEXTURE_RGBA8,
+    R8G8B8A8_UNORM_SRGB: abi.TEXTURE_RGBA8 | abi.TEXTURE_SRGB,
+}
 
 
 def rgb9e5_to_float(words: np.ndarray) -> np.ndarray:
@@ -60,10 +83,39 @@ def bc1_to_rgba8(blocks: np.ndarray, width: int, height: int) -> np.ndarray:
     return out[:height, :width]
 
 
-def read_dds(path) -> dict:
-    """{"width", "height", "depth", "format" (DXGI), "cubemap", "levels": [arrays]}:
-    each mip level decoded, RGBA8 uint8 [d?, h, w, 4] or float32 [d?, h, w, 3|4]
-    (the depth axis only for volume textures)."""
+def level_bytes(fmt: int, w: int, h: int, d: int = 1) -> int:
+    """Stored size of one level (one face) of a DXGI format."""
+    if fmt in (BC1_UNORM, BC1_UNORM_SRGB):
+        return ((w + 3) // 4) * ((h + 3) // 4) * 8 * d
+    if fmt in (BC7_UNORM, BC7_UNORM_SRGB):
+        return ((w + 3) // 4) * ((h + 3) // 4) * 16 * d
+    if fmt in (R8G8B8A8_UNORM, R8G8B8A8_UNORM_SRGB, R9G9B9E5_SHAREDEXP):
+        return w * h * d * 4
+    if fmt == R32G32B32A32_FLOAT:
+        return w * h * d * 16
+    raise ValueError(f"DXGI format {fmt} is not read")
+
+
+def _decode(fmt: int, buf: bytes, w: int, h: int, d: int):
+    if fmt in (BC1_UNORM, BC1_UNORM_SRGB):
+        n = level_bytes(fmt, w, h)
+        return np.stack([bc1_to_rgba8(np.frombuffer(buf, np.uint8, n, z * n), w, h) for z in range(d)])
+    if fmt in (R8G8B8A8_UNORM, R8G8B8A8_UNORM_SRGB):
+        return np.frombuffer(buf, np.uint8).reshape(d, h, w, 4).copy()
+    if fmt == R9G9B9E5_SHAREDEXP:
+        return rgb9e5_to_float(np.frombuffer(buf, np.uint32).reshape(d, h, w))
+    if fmt == R32G32B32A32_FLOAT:
+        return np.frombuffer(buf, np.float32).reshape(d, h, w, 4).copy()
+    raise ValueError(f"DXGI format {fmt} is not decoded on the host (read it with raw=True)")
+
+
+def read_dds(path, raw: bool = False) -> dict:
+    """{"width", "height", "depth", "format" (DXGI), "cubemap", "levels", "faces"}.
+
+    levels: each mip level of the first face - decoded (RGBA8 uint8 [d?, h, w, 4]
+    or float32 [d?, h, w, 3|4], the depth axis only for volume textures), or
+    with raw=True the stored bytes of each level (every format, BC7 included).
+    faces: for a cube map, the six faces' level lists (else [levels])."""
     data = Path(path).read_bytes()
     if data[:4] != b"DDS ":
         raise ValueError(f"{path}: not a DDS file")
@@ -73,46 +125,54 @@ def read_dds(path) -> dict:
     fourcc = data[4 + 80: 4 + 84]
     caps2 = struct.unpack_from("<I", data, 4 + 108)[0]
     offset = 4 + 124
+    array_size = 1
     if fourcc == b"DX10":
-        fmt = struct.unpack_from("<I", data, offset)[0]
+        fmt, _dim, _misc, array_size = struct.unpack_from("<4I", data, offset)
         offset += 20
     elif fourcc == b"DXT1":
         fmt = BC1_UNORM
     else:
         raise ValueError(f"{path}: unsupported pixel format {fourcc!r}")
+    cubemap = bool(caps2 & _DDSCAPS2_CUBEMAP)
     depth, mips = max(depth, 1), max(mips, 1)
-    levels, off = [], offset
-    for level in range(mips):
-        w, h, d = max(1, width >> level), max(1, height >> level), max(1, depth >> level)
-        if fmt in (BC1_UNORM, BC1_UNORM_SRGB):
-            n = ((w + 3) // 4) * ((h + 3) // 4) * 8
-            img = np.stack([bc1_to_rgba8(np.frombuffer(data, np.uint8, n, off + z * n), w, h) for z in range(d)])
-            off += n * d
-        elif fmt in (R8G8B8A8_UNORM, R8G8B8A8_UNORM_SRGB):
-            n = w * h * d * 4
-            img = np.frombuffer(data, np.uint8, n, off).reshape(d, h, w, 4).copy()
+    nfaces = 6 if cubemap else max(1, array_size)
+    faces, off = [], offset
+    for _ in range(nfaces):
+        levels = []
+        for level in range(mips):
+            w, h, d = max(1, width >> level), max(1, height >> level), max(1, depth >> level)
+            n = level_bytes(fmt, w, h, d)
+            if off + n > len(data):
+                raise ValueError(f"{path}: truncated at level {level}")
+            buf = data[off: off + n]
             off += n
-        elif fmt == R9G9B9E5_SHAREDEXP:
-            n = w * h * d
-            img = rgb9e5_to_float(np.frombuffer(data, np.uint32, n, off).reshape(d, h, w))
-            off += 4 * n
-        elif fmt == R32G32B32A32_FLOAT:
-            n = w * h * d * 4
-            img = np.frombuffer(data, np.float32, n, off).reshape(d, h, w, 4).copy()
-            off += 4 * n
-        else:
-            raise ValueError(f"{path}: DXGI format {fmt} is not read")
-        levels.append(img if depth > 1 else img[0])
-    return {"width": width, "height": height, "depth": depth, "format": fmt,
-            "cubemap": bool(caps2 & _DDSCAPS2_CUBEMAP), "levels": levels}
+            if raw:
+                levels.append(np.frombuffer(buf, np.uint8).copy())
+            else:
+                img = _decode(fmt, buf, w, h, d)
+                levels.append(img if depth > 1 else img[0])
+        faces.append(levels)
+    return {"width": width, "height": height, "depth": depth, "format": fmt, "cubemap": cubemap,
+            "levels": faces[0], "faces": faces}
+
+
+def load_dds_texture(path):
+    """A DDS file as the EncodedTexture the scene producer takes, with the
+    reference's format mapping (BC1 and BC7 sampled as sRGB)."""
+    from .scene import EncodedTexture
+
+    d = read_dds(path, raw=True)
+    if d["format"] not in TEXTURE_FORMAT:
+        raise ValueError(f"{path}: DXGI format {d['format']} is not a texture format of the kernels")
+    if d["cubemap"] or d["depth"] > 1:
+        raise ValueError(f"{path}: a 2D texture was expected")
+    return EncodedTexture(TEXTURE_FORMAT[d["format"]], d["width"], d["height"], d["levels"])
 
 
 def load_tonemap_lut(path) -> np.ndarray:
     """The display LUT of a 3D DDS (the reference's tony_mc_mapface.dds,
     48^3 RGB9E5) as the contiguous RGBA float32 [48, 48, 48, 4] array
     lh_view.tonemap_lut expects (red fastest, alpha 1)."""
-    from . import abi
-
     d = read_dds(path)
     n = abi.TONEMAP_LUT_DIMS
     lut = np.asarray(d["levels"][0], np.float32)
@@ -121,3 +181,18 @@ def load_tonemap_lut(path) -> np.ndarray:
     if lut.shape[3] == 3:
         lut = np.concatenate([lut, np.ones(lut.shape[:3] + (1,), np.float32)], -1)
     return np.ascontiguousarray(lut)
+
+
+def load_skybox(path) -> np.ndarray:
+    """A cube-map DDS (the reference's hdr-cubemap-1024x1024.dds, RGB9E5) as
+    the contiguous RGBA float32 [6, S, S, 4] array of level 0 that
+    lh_view.skybox expects (faces +X, -X, +Y, -Y, +Z, -Z; alpha 1)."""
+    d = read_dds(path)
+    if not d["cubemap"] or d["width"] != d["height"]:
+        raise ValueError(f"{path}: a square cube map was expected")
+    faces = [np.asarray(f[0], np.float32) for f in d["faces"]]
+    if faces[0].ndim != 3:
+        raise ValueError(f"{path}: float faces were expected (RGB9E5 or RGBA32F)")
+    faces = [f if f.shape[-1] == 4 else np.concatenate([f, np.ones(f.shape[:2] + (1,), np.float32)], -1)
+             for f in faces]
+    return np.ascontiguousarray(np.stack(faces))

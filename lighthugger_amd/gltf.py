@@ -11,15 +11,25 @@ What is read:
     KHR_mesh_quantization allows (float, (unsigned) byte / short, normalized
     or not; indices u8 / u16 / u32), dequantized by the glTF rules
     (normalized: c / 255, c / 65535, max(c / 127, -1), max(c / 32767, -1));
-  * the node hierarchy (matrix, or translation * rotation * scale), each
-    (node, primitive) pair becoming one instance with the node's world matrix
-    (KHR_mesh_quantization puts the position dequantization there);
+  * node transforms, node_transforms="gltf" (default): the glTF rules - the
+    node's matrix, or translation * rotation * scale, composed down the
+    hierarchy of the chosen scene, each (node, primitive) pair one instance
+    with the node's world matrix (KHR_mesh_quantization puts the position
+    dequantization there); node_transforms="reference": what the reference's
+    NodeTree computes (src/resources/mesh_loading.cpp:16-47, 356-368) - every
+    node with a mesh, in node order, whether or not a scene lists it, with its
+    own scale(S) * translate(T) (rotation, matrices and parents ignored);
   * materials: base-colour factor, roughness and metallic factors,
-    alphaMode MASK (cutoff 0.5, as the reference requires) -> alpha clip,
-    KHR_texture_transform scale / offset of the base-colour texture;
-  * images only when the caller passes them decoded (`images`: glTF image
-    index -> RGBA8 array H x W x 4): this build decodes neither PNG/JPEG nor
-    KTX2/DDS.  A texture whose image is not supplied is left unused.
+    alphaMode MASK (cutoff 0.5, as the reference requires) -> alpha clip; the
+    base-colour, metallic-roughness and normal textures, each with its
+    KHR_texture_transform scale / offset (the last one present wins, as in
+    the reference, src/resources/mesh_loading.cpp:495-550);
+  * images: .ktx2 and .dds URIs are loaded as stored (ktx2.py, dds.py: BC1,
+    BC7, RGBA8, sRGB as the file / the reference says), decoded on fetch by
+    the kernels; other images (PNG / JPEG) only when the caller passes them
+    decoded (`images`: glTF image index -> RGBA8 array H x W x 4, or an
+    EncodedTexture), which also overrides a file.  A texture whose image is
+    neither is left unused.
 
 The producer then builds meshlets and bounds, and writes the vertices in
 the requested vertex format (FP32, or the reference's quantized formats).
@@ -34,7 +44,9 @@ from pathlib import Path
 import numpy as np
 
 from . import abi
-from .scene import HostScene, Material
+from .dds import load_dds_texture
+from .ktx2 import load_ktx2_texture
+from .scene import EncodedTexture, HostScene, Material
 
 _COMPONENT = {5120: np.int8, 5121: np.uint8, 5122: np.int16, 5123: np.uint16, 5125: np.uint32, 5126: np.float32}
 _COMPONENTS = {"SCALAR": 1, "VEC2": 2, "VEC3": 3, "VEC4": 4, "MAT4": 16}
@@ -126,11 +138,14 @@ def _material(doc: dict, index, texture_of_image: dict) -> Material:
     mat = Material(base_color=tuple(pbr.get("baseColorFactor", [1, 1, 1, 1])[:3]),
                    roughness=float(pbr.get("roughnessFactor", 1.0)), metallic=float(pbr.get("metallicFactor", 1.0)),
                    alpha_clip=mode == "MASK")
-    tex = pbr.get("baseColorTexture")
-    if tex is not None:
+    for field, tex in (("base_color_texture", pbr.get("baseColorTexture")),
+                       ("metallic_roughness_texture", pbr.get("metallicRoughnessTexture")),
+                       ("normal_texture", m.get("normalTexture"))):
+        if tex is None:
+            continue
         image = doc["textures"][tex["index"]].get("source")
         if image in texture_of_image:
-            mat.base_color_texture = texture_of_image[image]
+            setattr(mat, field, texture_of_image[image])
         tt = tex.get("extensions", {}).get("KHR_texture_transform")
         if tt is not None:
             mat.texture_scale = tuple(tt.get("scale", [1.0, 1.0]))
@@ -138,17 +153,50 @@ def _material(doc: dict, index, texture_of_image: dict) -> Material:
     return mat
 
 
+def _reference_node_matrix(node: dict) -> np.ndarray:
+    """Row-major scale(S) * translate(T) of a TRS node (identity for a matrix
+    node), as the reference's NodeTree forms it (mesh_loading.cpp:16-35)."""
+    if "matrix" in node:
+        return np.eye(4)
+    t = np.asarray(node.get("translation", [0, 0, 0]), np.float64)
+    s = np.asarray(node.get("scale", [1, 1, 1]), np.float64)
+    m = np.eye(4)
+    m[:3, :3] = np.diag(s)
+    m[:3, 3] = s * t
+    return m
+
+
+def _image_texture(path: Path, image: dict):
+    uri = image.get("uri")
+    if uri is None or uri.startswith("data:"):
+        return None
+    suffix = Path(uri).suffix.lower()
+    if suffix == ".ktx2":
+        return load_ktx2_texture(path.parent / uri)
+    if suffix == ".dds":
+        return load_dds_texture(path.parent / uri)
+    return None
+
+
 def load_gltf(path, *, vertex_format: int = abi.VERTEX_FORMAT_FP32, images: dict | None = None,
-              scene: int | None = None) -> HostScene:
+              scene: int | None = None, node_transforms: str = "gltf") -> HostScene:
     """A HostScene of the glTF file's default (or given) scene; see the module
     docstring for what is read."""
+    if node_transforms not in ("gltf", "reference"):
+        raise ValueError(f"node_transforms {node_transforms!r}: 'gltf' or 'reference'")
     path = Path(path)
     doc, buffers = _read_container(path)
-    images = images or {}
+    supplied = images or {}
     textures, texture_of_image = [], {}
-    for image, rgba in sorted(images.items()):
-        texture_of_image[image] = len(textures)
-        textures.append(np.ascontiguousarray(rgba, np.uint8))
+    for i, img in enumerate(doc.get("images", [])):
+        tex = supplied.get(i)
+        if tex is None:
+            tex = _image_texture(path, img)
+        elif not isinstance(tex, EncodedTexture):
+            tex = np.ascontiguousarray(tex, np.uint8)
+        if tex is not None:
+            texture_of_image[i] = len(textures)
+            textures.append(tex)
     meshes, mesh_of, materials, material_of, instances = [], {}, [], {}, []
 
     def primitive_mesh(mi: int, pi: int) -> int:
@@ -168,8 +216,13 @@ def load_gltf(path, *, vertex_format: int = abi.VERTEX_FORMAT_FP32, images: dict
         return mesh_of[key]
 
     def walk(node_index: int, parent: np.ndarray) -> None:
+        world = parent @ _local_matrix(doc["nodes"][node_index])
+        walk_one(node_index, world)
+        for child in doc["nodes"][node_index].get("children", []):
+            walk(child, world)
+
+    def walk_one(node_index: int, world: np.ndarray) -> None:
         node = doc["nodes"][node_index]
-        world = parent @ _local_matrix(node)
         if "mesh" in node:
             for pi, prim in enumerate(doc["meshes"][node["mesh"]]["primitives"]):
                 if prim.get("mode", 4) != 4:
@@ -180,14 +233,17 @@ def load_gltf(path, *, vertex_format: int = abi.VERTEX_FORMAT_FP32, images: dict
                     materials.append(_material(doc, mat, texture_of_image))
                 instances.append((primitive_mesh(node["mesh"], pi), material_of[mat],
                                   np.ascontiguousarray(world.T, np.float32)))  # column-major
-        for child in node.get("children", []):
-            walk(child, world)
 
     scenes = doc.get("scenes", [])
     s = scene if scene is not None else doc.get("scene", 0)
-    roots = scenes[s]["nodes"] if scenes else list(range(len(doc.get("nodes", []))))
-    for r in roots:
-        walk(r, np.eye(4))
+    if node_transforms == "reference":
+        for ni, node in enumerate(doc.get("nodes", [])):
+            if "mesh" in node:
+                walk_one(ni, _reference_node_matrix(node))
+    else:
+        roots = scenes[s]["nodes"] if scenes else list(range(len(doc.get("nodes", []))))
+        for r in roots:
+            walk(r, np.eye(4))
     if not instances:
         raise ValueError(f"{path}: no triangle primitives in scene {s}")
     return HostScene.build(meshes, materials, instances, textures=textures, vertex_format=vertex_format)

@@ -33,6 +33,9 @@ def scene_lib() -> C.CDLL:
                                           C.c_uint32]
         lib.lh_scene_add_texture.restype = C.c_int32
         lib.lh_scene_add_texture.argtypes = [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32]
+        lib.lh_scene_add_texture_encoded.restype = C.c_int32
+        lib.lh_scene_add_texture_encoded.argtypes = [C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32,
+                                                     C.c_void_p, C.c_uint64]
         lib.lh_scene_add_material.restype = C.c_int32
         lib.lh_scene_add_material.argtypes = [C.c_void_p, abi.Vec3, C.c_float, C.c_float, C.c_uint32, C.c_uint32,
                                               C.c_uint32, abi.Vec2, abi.Vec2]
@@ -112,6 +115,18 @@ def make_transform(translation, yaw: float, scale: float) -> np.ndarray:
 
 
 @dataclass
+class EncodedTexture:
+    """A texture as a file stores it: `levels` holds the bytes of each mip
+    level (level 0 first) in `format` (abi.TEXTURE_RGBA8 / BC1 / BC7, optionally
+    | abi.TEXTURE_SRGB); block formats hold ceil(w/4) x ceil(h/4) blocks."""
+
+    format: int
+    width: int
+    height: int
+    levels: list
+
+
+@dataclass
 class Material:
     base_color: tuple = (1.0, 1.0, 1.0)
     roughness: float = 1.0
@@ -172,8 +187,9 @@ class HostScene:
               texture_format=abi.TEXTURE_RGBA8, cache_dir=None) -> "HostScene":
         """meshes: [(positions[N,3], normals[N,3] | None, uvs[N,2] | None, indices[T,3])];
         materials: [Material]; instances: [(mesh, material, transform[4,4] column-major)];
-        textures: [uint8 RGBA array (H, W, 4)]; vertex_format: abi.VERTEX_FORMAT_*;
-        texture_format: abi.TEXTURE_*; cache_dir: directory of the built-mesh cache or None."""
+        textures: [uint8 RGBA array (H, W, 4) | EncodedTexture]; vertex_format: abi.VERTEX_FORMAT_*;
+        texture_format: abi.TEXTURE_* (| abi.TEXTURE_SRGB), the storage of the RGBA textures;
+        cache_dir: directory of the built-mesh cache or None."""
         lib = scene_lib()
         h = lib.lh_scene_create()
         try:
@@ -194,6 +210,13 @@ class HostScene:
                 if r < 0:
                     raise ValueError("lh_scene_add_mesh failed")
             for tex in textures:
+                if isinstance(tex, EncodedTexture):
+                    data = np.ascontiguousarray(np.concatenate([np.frombuffer(bytes(lv), np.uint8)
+                                                                for lv in tex.levels]))
+                    if lib.lh_scene_add_texture_encoded(h, tex.format, tex.width, tex.height, len(tex.levels),
+                                                        data.ctypes.data, data.nbytes) < 0:
+                        raise ValueError("lh_scene_add_texture_encoded refused the texture")
+                    continue
                 tex = np.ascontiguousarray(tex, np.uint8)
                 if lib.lh_scene_add_texture(h, tex.ctypes.data, tex.shape[1], tex.shape[0]) < 0:
                     raise ValueError("lh_scene_add_texture failed")

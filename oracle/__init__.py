@@ -24,46 +24,66 @@ def lib() -> C.CDLL:
             from . import build  # noqa: PLC0415
 
             build.build()
-        L = C.CDLL(str(path))
-        for name in _VIEW_FNS:
-            getattr(L, name).restype = C.c_int32
-            getattr(L, name).argtypes = [C.c_void_p, C.c_void_p]
-        L.orc_cull_meshlets.restype = C.c_int32
-        L.orc_cull_meshlets.argtypes = [C.c_void_p, C.c_void_p, C.c_uint32]
-        L.orc_raster_shadow.restype = C.c_int32
-        L.orc_raster_shadow.argtypes = [C.c_void_p, C.c_void_p, C.c_uint32]
-        L.orc_tonemap.restype = C.c_int32
-        L.orc_tonemap.argtypes = [C.c_void_p]
-        L.orc_raster_clip_triangle.restype = C.c_uint64
-        L.orc_raster_clip_triangle.argtypes = [C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32,
-                                               C.c_void_p, C.c_void_p]
-        L.orc_cull_sphere.restype = C.c_int
-        L.orc_cull_sphere.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
-        L.orc_cull_sphere_shadow.restype = C.c_int
-        L.orc_cull_sphere_shadow.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32]
-        L.orc_cull_cone.restype = C.c_int
-        L.orc_cull_cone.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int]
-        L.orc_calc_full_bary.restype = None
-        L.orc_calc_full_bary.argtypes = [C.c_void_p] * 6
-        L.orc_brdf.restype = None
-        L.orc_brdf.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_float, C.c_float, C.c_void_p, C.c_int,
-                               C.c_void_p]
-        L.orc_srgb_oetf.restype = C.c_float
-        L.orc_srgb_oetf.argtypes = [C.c_float]
-        L.orc_frame_threads.restype = C.c_int32
-        L.orc_frame_threads.argtypes = [C.c_void_p, C.c_void_p, C.c_uint32]
-        L.orc_bc1_texel.restype = C.c_uint32
-        L.orc_bc1_texel.argtypes = [C.c_void_p, C.c_uint32, C.c_uint32]
-        L.orc_hiz_bound_violations.restype = C.c_int64
-        L.orc_hiz_bound_violations.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
-        L.orc_resolve_frozen.restype = C.c_int32
-        L.orc_resolve_frozen.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
-        L.orc64_resolve.restype = C.c_int32
-        L.orc64_resolve.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
-        L.orc64_resolve_grad.restype = C.c_int32
-        L.orc64_resolve_grad.argtypes = [C.c_void_p] * 5 + [C.c_void_p] * 6
-        _lib = L
+        _lib = bind(C.CDLL(str(path)))
     return _lib
+
+
+def bind(L: C.CDLL) -> C.CDLL:
+    """Declares the argument and result types of the oracle's exports on L."""
+    for name in _VIEW_FNS:
+        getattr(L, name).restype = C.c_int32
+        getattr(L, name).argtypes = [C.c_void_p, C.c_void_p]
+    L.orc_cull_meshlets.restype = C.c_int32
+    L.orc_cull_meshlets.argtypes = [C.c_void_p, C.c_void_p, C.c_uint32]
+    L.orc_raster_shadow.restype = C.c_int32
+    L.orc_raster_shadow.argtypes = [C.c_void_p, C.c_void_p, C.c_uint32]
+    L.orc_tonemap.restype = C.c_int32
+    L.orc_tonemap.argtypes = [C.c_void_p]
+    L.orc_raster_clip_triangle.restype = C.c_uint64
+    L.orc_raster_clip_triangle.argtypes = [C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32,
+                                           C.c_void_p, C.c_void_p]
+    L.orc_cull_sphere.restype = C.c_int
+    L.orc_cull_sphere.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
+    L.orc_cull_sphere_shadow.restype = C.c_int
+    L.orc_cull_sphere_shadow.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32]
+    L.orc_cull_cone.restype = C.c_int
+    L.orc_cull_cone.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int]
+    L.orc_calc_full_bary.restype = None
+    L.orc_calc_full_bary.argtypes = [C.c_void_p] * 6
+    L.orc_brdf.restype = None
+    L.orc_brdf.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_float, C.c_float, C.c_void_p, C.c_int,
+                           C.c_void_p]
+    L.orc_srgb_oetf.restype = C.c_float
+    L.orc_srgb_oetf.argtypes = [C.c_float]
+    L.orc_frame_threads.restype = C.c_int32
+    L.orc_frame_threads.argtypes = [C.c_void_p, C.c_void_p, C.c_uint32]
+    L.orc_bc1_texel.restype = C.c_uint32
+    L.orc_bc1_texel.argtypes = [C.c_void_p, C.c_uint32, C.c_uint32]
+    L.orc_hiz_bound_violations.restype = C.c_int64
+    L.orc_hiz_bound_violations.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
+    L.orc_resolve_frozen.restype = C.c_int32
+    L.orc_resolve_frozen.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
+    L.orc64_resolve.restype = C.c_int32
+    L.orc64_resolve.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+    L.orc64_resolve_grad.restype = C.c_int32
+    L.orc64_resolve_grad.argtypes = [C.c_void_p] * 5 + [C.c_void_p] * 6
+    L.orc_bc7_block.restype = None
+    L.orc_bc7_block.argtypes = [C.c_void_p, C.c_void_p]
+    L.orc_srgb_to_linear.restype = C.c_float
+    L.orc_srgb_to_linear.argtypes = [C.c_uint32]
+    L.orc_shadow_tap.restype = C.c_float
+    L.orc_shadow_tap.argtypes = [C.c_void_p, C.c_uint32, C.c_void_p, C.c_float]
+    L.orc_shadow_factor.restype = C.c_float
+    L.orc_shadow_factor.argtypes = [C.c_void_p] * 4
+    L.orc_texture_lod.restype = None
+    L.orc_texture_lod.argtypes = [C.c_void_p] * 5
+    L.orc_bilinear.restype = None
+    L.orc_bilinear.argtypes = [C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p]
+    L.orc_sample_texture.restype = None
+    L.orc_sample_texture.argtypes = [C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p]
+    L.orc_normal_map.restype = None
+    L.orc_normal_map.argtypes = [C.c_void_p] * 5
+    return L
 
 
 def check(status: int, what: str = "oracle") -> None:

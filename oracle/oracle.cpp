@@ -556,7 +556,8 @@ Interp2 interp2(const BaryDeriv& d, F2 a, F2 b, F2 c) {
     return {{x.x, y.x}, {x.y, y.y}, {x.z, y.z}};
 }
 
-// RGBA8 UNORM texel fetch with repeat addressing.
+// Texel fetch with repeat addressing: RGBA8, BC1 or BC7 storage, sRGB decoded
+// to linear when the format says so.
 F4 texel(const LhTexture& t, uint32_t level, int32_t i, int32_t j) {
     int32_t w = int32_t(std::max(1u, t.width >> level)), h = int32_t(std::max(1u, t.height >> level));
     i %= w;
@@ -565,6 +566,9 @@ F4 texel(const LhTexture& t, uint32_t level, int32_t i, int32_t j) {
     if (j < 0) j += h;
     const orc_texel::RGBA8 c =
         orc_texel::fetch(t, at<uint8_t>(t.texels), level, uint32_t(w), uint32_t(i), uint32_t(j));
+    if (t.format & LH_TEXTURE_SRGB)
+        return {orc_texel::srgb_to_linear<float>(c.c[0]), orc_texel::srgb_to_linear<float>(c.c[1]),
+                orc_texel::srgb_to_linear<float>(c.c[2]), c.c[3] / 255.0f};
     return {c.c[0] / 255.0f, c.c[1] / 255.0f, c.c[2] / 255.0f, c.c[3] / 255.0f};
 }
 F4 mix4(F4 a, F4 b, float t) { return {mixf(a.x, b.x, t), mixf(a.y, b.y, t), mixf(a.z, b.z, t), mixf(a.w, b.w, t)}; }
@@ -748,6 +752,24 @@ float shadow_factor(const lh_view& v, const MeshInfo& mi, F3 world, uint32_t* ca
     return shadow_sum;
 }
 
+// Normal mapping (render_geometry.comp:238-246): the sampled texel decoded
+// as map = texel * 255 / 127 - 128 / 127, rotated by the cotangent frame of
+// compute_cotangent_frame (:25-38) - the screen-space derivatives of the
+// world position and uv give T = dp/dy_perp * du/dx + dp/dx_perp * du/dy and
+// B likewise for v, both scaled by 1 / max(|T|, |B|) - and normalized.
+F3 normal_map(F3 nrm, const Interp3& world, const Interp2& uv, F4 mn) {
+    F3 map{mn.x * 255.0f / 127.0f - 128.0f / 127.0f, mn.y * 255.0f / 127.0f - 128.0f / 127.0f,
+           mn.z * 255.0f / 127.0f - 128.0f / 127.0f};
+    F3 dpy_perp = cross3(world.dy, nrm), dpx_perp = cross3(nrm, world.dx);
+    F3 t = add(mul(dpy_perp, uv.dx.x), mul(dpx_perp, uv.dy.x));
+    F3 b = add(mul(dpy_perp, uv.dx.y), mul(dpx_perp, uv.dy.y));
+    float invmax = 1.0f / std::sqrt(std::max(dot3(t, t), dot3(b, b)));
+    t = mul(t, invmax);
+    b = mul(b, invmax);
+    return normalize3(F3{(t.x * map.x + b.x * map.y) + nrm.x * map.z, (t.y * map.x + b.y * map.y) + nrm.y * map.z,
+                         (t.z * map.x + b.z * map.y) + nrm.z * map.z});
+}
+
 // DEBUG_COLOURS (src/shaders/common/debug.glsl:1-11).
 F3 debug_colour(uint32_t i) {
     static const float k[10][3] = {{0, 2, 91},    {0, 108, 251}, {0, 221, 221}, {51, 221, 0},  {255, 252, 0},
@@ -805,20 +827,8 @@ F3 resolve_pixel(const lh_scene& sc, const lh_view& v, uint32_t x, uint32_t y) {
     F4 mr = sample_texture(sc, mi.metallic_roughness_texture_index, uv);
     float roughness = mr.y * fac.roughness_factor;
     float metallic = mr.z * fac.metallic_factor;
-    if (mi.normal_texture_index != UNUSED_TEXTURE_INDEX) {
-        F4 mn = sample_texture(sc, mi.normal_texture_index, uv);
-        F3 map{mn.x * 255.0f / 127.0f - 128.0f / 127.0f, mn.y * 255.0f / 127.0f - 128.0f / 127.0f,
-               mn.z * 255.0f / 127.0f - 128.0f / 127.0f};
-        // compute_cotangent_frame (:25-38)
-        F3 dpy_perp = cross3(world.dy, nrm), dpx_perp = cross3(nrm, world.dx);
-        F3 t = add(mul(dpy_perp, uv.dx.x), mul(dpx_perp, uv.dy.x));
-        F3 b = add(mul(dpy_perp, uv.dx.y), mul(dpx_perp, uv.dy.y));
-        float invmax = 1.0f / std::sqrt(std::max(dot3(t, t), dot3(b, b)));
-        t = mul(t, invmax);
-        b = mul(b, invmax);
-        nrm = normalize3(F3{(t.x * map.x + b.x * map.y) + nrm.x * map.z, (t.y * map.x + b.y * map.y) + nrm.y * map.z,
-                            (t.z * map.x + b.z * map.y) + nrm.z * map.z});
-    }
+    if (mi.normal_texture_index != UNUSED_TEXTURE_INDEX)
+        nrm = normal_map(nrm, world, uv, sample_texture(sc, mi.normal_texture_index, uv));
     // Debug views (:268-308); the shader-clock heat map is not reproduced.
     if (u.debug == UNIFORMS_DEBUG_CASCADES) return mul(base, debug_colour(cascade));
     if (u.debug == UNIFORMS_DEBUG_TRIANGLE_INDEX) return debug_colour(tri % 10u);
@@ -1332,5 +1342,43 @@ void orc_brdf(const float* v, const float* l, const float* n, float r, float m, 
     o[2] = x.z;
 }
 float orc_srgb_oetf(float x) { return srgb_oetf(x); }
+
+void orc_bc7_block(const uint8_t* block, uint8_t* rgba64) {
+    orc_texel::RGBA8 t[16];
+    orc_texel::bc7::decode(block, t);
+    for (int k = 0; k < 16; k++) std::memcpy(rgba64 + 4 * k, t[k].c, 4);
+}
+float orc_srgb_to_linear(uint32_t v) { return orc_texel::srgb_to_linear<float>(uint8_t(v)); }
+
+// ---- shading building blocks (known-answer tests) ----
+float orc_shadow_tap(const lh_view* v, uint32_t layer, const float* uv, float ref) {
+    return shadow_tap(*v, layer, F2{uv[0], uv[1]}, ref);
+}
+float orc_shadow_factor(const lh_view* v, const MeshInfo* mi, const float* world, uint32_t* cascade) {
+    uint32_t c = 0;
+    float r = shadow_factor(*v, *mi, F3{world[0], world[1], world[2]}, &c);
+    if (cascade) *cascade = c;
+    return r;
+}
+static Interp2 uv_of(const float* uv6) { return {{uv6[0], uv6[1]}, {uv6[2], uv6[3]}, {uv6[4], uv6[5]}}; }
+void orc_texture_lod(const LhTexture* t, const float* uv6, uint32_t* l0, uint32_t* l1, float* f) {
+    Lod lod = texture_lod(*t, uv_of(uv6));
+    *l0 = lod.l0;
+    *l1 = lod.l1;
+    *f = lod.f;
+}
+void orc_bilinear(const LhTexture* t, uint32_t level, const float* uv, float* rgba) {
+    F4 c = bilinear(*t, level, F2{uv[0], uv[1]});
+    rgba[0] = c.x, rgba[1] = c.y, rgba[2] = c.z, rgba[3] = c.w;
+}
+void orc_sample_texture(const lh_scene* sc, uint32_t index, const float* uv6, float* rgba) {
+    F4 c = sample_texture(*sc, index, uv_of(uv6));
+    rgba[0] = c.x, rgba[1] = c.y, rgba[2] = c.z, rgba[3] = c.w;
+}
+void orc_normal_map(const float* n, const float* world9, const float* uv6, const float* texel, float* out) {
+    Interp3 w{{world9[0], world9[1], world9[2]}, {world9[3], world9[4], world9[5]}, {world9[6], world9[7], world9[8]}};
+    F3 r = normal_map(F3{n[0], n[1], n[2]}, w, uv_of(uv6), F4{texel[0], texel[1], texel[2], texel[3]});
+    out[0] = r.x, out[1] = r.y, out[2] = r.z;
+}
 
 }  // extern "C"

@@ -73,6 +73,30 @@ float orc_srgb_oetf(float x);
 int64_t orc_hiz_bound_violations(const lh_scene* scene, const lh_view* view, uint64_t* checked);
 /* One texel (x, y) of a BC1 block (8 bytes), packed RGBA8 with red in the low byte. */
 uint32_t orc_bc1_texel(const uint8_t* block, uint32_t x, uint32_t y);
+/* The 16 texels of a BC7 block (16 bytes) as RGBA8, row order (64 bytes). */
+void orc_bc7_block(const uint8_t* block, uint8_t* rgba64);
+/* sRGB EOTF of an 8-bit channel value (the *_SRGB sampler decode). */
+float orc_srgb_to_linear(uint32_t v);
+
+/* Shading building blocks of the resolve (render_geometry.comp), exposed so
+ * that closed forms can pin them:
+ *   orc_shadow_tap     one sampler2DArrayShadow tap (linear filter, clamp to
+ *                      edge, compare LESS) of layer `layer` at uv[2] against ref;
+ *   orc_shadow_factor  cascade selection + 3x3 PCF (:183-226) of a world
+ *                      point; *cascade gets the loop's end index (0..4);
+ *   orc_texture_lod    mip levels and blend of the trilinear filter from
+ *                      uv6 = {u, v, du/dx, dv/dx, du/dy, dv/dy};
+ *   orc_bilinear       bilinear fetch of one level with repeat addressing;
+ *   orc_sample_texture trilinear sample_texture (:59-66) of scene texture `index`;
+ *   orc_normal_map     the normal-map step (:238-246) with the cotangent frame
+ *                      (:25-38): normal[3], world9 = {p, dp/dx, dp/dy},
+ *                      uv6 as above, texel[4] the sampled RGBA in [0, 1]. */
+float orc_shadow_tap(const lh_view* view, uint32_t layer, const float* uv, float ref);
+float orc_shadow_factor(const lh_view* view, const MeshInfo* mesh_info, const float* world, uint32_t* cascade);
+void orc_texture_lod(const LhTexture* texture, const float* uv6, uint32_t* l0, uint32_t* l1, float* f);
+void orc_bilinear(const LhTexture* texture, uint32_t level, const float* uv, float* rgba);
+void orc_sample_texture(const lh_scene* scene, uint32_t index, const float* uv6, float* rgba);
+void orc_normal_map(const float* normal, const float* world9, const float* uv6, const float* texel, float* out3);
 
 /* ---- differentiable resolve ----
  * Per-pixel quantities the adjoint holds fixed (FP32, as the forward resolve

@@ -188,7 +188,8 @@ uint32_t load_index(const MeshInfo& mi, uint32_t i) {
     return at<uint16_t>(mi.indices)[i];
 }
 
-// Texel of an RGBA8 level with repeat addressing, as doubles in [0, 1].
+// Texel of a level with repeat addressing, as doubles in [0, 1] (sRGB
+// formats decoded to linear).
 V4<double> texel(const LhTexture& t, uint32_t level, int64_t i, int64_t j) {
     const int64_t w = std::max<int64_t>(1, t.width >> level), h = std::max<int64_t>(1, t.height >> level);
     i %= w;
@@ -198,6 +199,9 @@ V4<double> texel(const LhTexture& t, uint32_t level, int64_t i, int64_t j) {
     const orc_texel::RGBA8 c =
         orc_texel::fetch(t, at<uint8_t>(t.texels), level, uint32_t(w), uint32_t(i), uint32_t(j));
     const uint8_t* q = c.c;
+    if (t.format & LH_TEXTURE_SRGB)
+        return {orc_texel::srgb_to_linear<double>(q[0]), orc_texel::srgb_to_linear<double>(q[1]),
+                orc_texel::srgb_to_linear<double>(q[2]), q[3] / 255.0};
     return {q[0] / 255.0, q[1] / 255.0, q[2] / 255.0, q[3] / 255.0};
 }
 template <class T>

@@ -12,7 +12,7 @@ import numpy as np
 import pytest
 
 from lighthugger_amd import abi
-from lighthugger_amd.scene import HostScene, make_camera, make_uniforms, preset_camera
+from lighthugger_amd.scene import HostScene, Material, make_camera, make_uniforms, preset_camera
 
 from .util import cube_mesh, grid_patch, single_scene, translate
 
@@ -254,3 +254,46 @@ def test_built_mesh_cache(tmp_path, vertex_format):
     other = abi.VERTEX_FORMAT_QUANTIZED if vertex_format == abi.VERTEX_FORMAT_FP32 else abi.VERTEX_FORMAT_FP32
     HostScene.preset(abi.PRESET_LARGE, cache_dir=tmp_path, **{**kw, "vertex_format": other})
     assert len(list(tmp_path.glob("lhmesh-*.bin"))) == 2 * len(files)
+
+
+def _fnv1a64(data: bytes) -> int:
+    h = 1469598103934665603
+    for b in data:
+        h = ((h ^ b) * 1099511628211) & 0xFFFFFFFFFFFFFFFF
+    return h
+
+
+@pytest.mark.parametrize("damage", ["micro_index", "triangle_count", "vertex_count"])
+def test_built_mesh_cache_rejects_meshlets_beyond_the_raster_limits(tmp_path, damage):
+    """A cache file that passes its checksum but names a micro index at or
+    beyond its meshlet's vertex count, or a meshlet over 124 triangles / 64
+    vertices, is ignored and the mesh rebuilt (the raster assumes those
+    limits, the resolve indexes the remap table with the micro indices)."""
+    import struct
+
+    mesh = [grid_patch(12, 12)]
+    mat = [Material(base_color=(0.5, 0.5, 0.5))]
+    inst = [(0, 0, np.eye(4, dtype=np.float32))]
+    plain = HostScene.build(mesh, mat, inst)
+    HostScene.build(mesh, mat, inst, cache_dir=tmp_path)
+    (path,) = tmp_path.glob("lhmesh-*.bin")
+    raw = bytearray(path.read_bytes())
+    off = 8  # magic, version
+    sizes = {}
+    for name, item in (("pos", 4), ("nrm", 4), ("uv", 4), ("idx", 4), ("meshlets", 56), ("remap", 4), ("micro", 1)):
+        (n,) = struct.unpack_from("<Q", raw, off)
+        sizes[name] = (off + 8, n)
+        off += 8 + n * item
+    ml0 = sizes["meshlets"][0]
+    tri_off, _idx_off, ntri, nvert = struct.unpack_from("<IIBB", raw, ml0 + 44)
+    if damage == "micro_index":
+        raw[sizes["micro"][0] + tri_off + 1] = nvert  # one past the meshlet's vertices
+    elif damage == "triangle_count":
+        raw[ml0 + 52] = 125
+    else:
+        raw[ml0 + 53] = 65
+    raw[-8:] = struct.pack("<Q", _fnv1a64(bytes(raw[:-8])))
+    path.write_bytes(bytes(raw))
+    again = HostScene.build(mesh, mat, inst, cache_dir=tmp_path)
+    for k in ("geometry", "mesh_infos", "meshlet_table"):
+        assert np.array_equal(getattr(again, k), getattr(plain, k)), k
