@@ -41,9 +41,14 @@ def oracle_view(host_scene, cfg: ViewConfig, uniforms: np.ndarray):
     return sd, vd
 
 
-def oracle_frame(host_scene, cfg: ViewConfig, uniforms: np.ndarray) -> dict:
+def oracle_frame(host_scene, cfg: ViewConfig, uniforms: np.ndarray, threads: int = 1) -> dict:
+    """The whole frame on the CPU; threads > 1 splits the rasters and the
+    resolve over host threads (identical outputs)."""
     sd, vd = oracle_view(host_scene, cfg, uniforms)
-    st = oracle_lib().orc_frame(sd.address, vd.address)
+    if threads > 1:
+        st = oracle_lib().orc_frame_threads(sd.address, vd.address, threads)
+    else:
+        st = oracle_lib().orc_frame(sd.address, vd.address)
     if st not in (abi.LH_OK, abi.LH_ERR_CAPACITY):
         raise RuntimeError(f"oracle frame failed: {st}")
     return snapshot(vd)

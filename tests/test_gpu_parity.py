@@ -140,10 +140,14 @@ def test_tonemap_lut_matches_oracle():
     assert (gpu["ldr"] != orc["ldr"]).mean() < 1e-3
 
 
-def test_bc1_texture_frame_parity():
-    """The benchmark workload at full size with every texture in BC1, decoded
-    on fetch by the resolve: keys bit-exact, radiance at the usual bar."""
-    scene = HostScene.preset(abi.PRESET_LARGE, texture_format=abi.TEXTURE_BC1)
+@pytest.mark.parametrize("texture_format", [abi.TEXTURE_BC1, abi.TEXTURE_BC1 | abi.TEXTURE_SRGB,
+                                            abi.TEXTURE_BC7 | abi.TEXTURE_SRGB, abi.TEXTURE_RGBA8 | abi.TEXTURE_SRGB],
+                         ids=["bc1", "bc1_srgb", "bc7_srgb", "rgba8_srgb"])
+def test_block_texture_frame_parity(texture_format):
+    """The benchmark workload at full size with every texture stored in the
+    given format (BC1 / BC7 blocks decoded on fetch, sRGB decoded per texel
+    before filtering): keys bit-exact, radiance at the usual bar."""
+    scene = HostScene.preset(abi.PRESET_LARGE, texture_format=texture_format)
     cam = preset_camera(abi.PRESET_LARGE)
     rep = _run(scene, cam, ViewConfig(cam.width, cam.height, 2048, big_capacity=1 << 21))
     assert rep["covered_pixels"] > 100000
