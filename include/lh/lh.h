@@ -57,6 +57,9 @@ typedef void* lh_stream; /* hipStream_t */
 #define LH_CNT_ATOMICS 40   /* + pass: depth atomics issued (same; main view: samples with depth > 0) */
 #define LH_CNT_ALPHA_DRAWS 48 /* + pass: surviving meshlets of alpha-clip mesh infos (with LH_VIEW_HIZ: before
                                  the occlusion test); zero skips the alpha-clip raster of the pass */
+#define LH_CNT_TILE_ITEMS 56   /* tiled main-view raster: work items (tile, run of binned meshlets) of the last launch */
+#define LH_CNT_TILE_DIRECT 58  /* tiled main-view raster: meshlets left to the per-meshlet raster (beyond
+                                  bin_capacity, or a screen rectangle over LH_TILE_MAX_RECT tiles) */
 #define LH_NUM_COUNTERS 64
 #define LH_OVERFLOW_INSTANCES 0x1u
 #define LH_OVERFLOW_DRAWS 0x2u
@@ -76,10 +79,23 @@ typedef void* lh_stream; /* hipStream_t */
                                    rasterized, and the candidates are staged in draws[1] (free until
                                    the shadow passes). */
 
-/* Bytes per entry of the two raster queues inside lh_view.scratch:
- * [clip queue: clip_capacity x 8 B][pad to 256][big queue: big_capacity x 48 B]. */
+/* Raster scratch inside lh_view.scratch, each part 256-byte aligned, sizes from
+ * lh_scratch_bytes(view):
+ *   clip queue     clip_capacity x 8 B
+ *   big queue      big_capacity x 48 B
+ *   tiled main-view raster (only when bin_capacity > 0):
+ *     bin headers  bin_capacity x 16 B      (screen rectangle, triangle count, meshlet id per draw)
+ *     bin cache    bin_capacity x 1280 B    (snapped vertices and front-facing triangles per draw)
+ *     tile counts  2 x tiles x 4 B          (tiles = ceil(width / 64) * ceil(height / 64))
+ *     items        (tiles + entries / 8) x 12 B
+ *     entries      LH_TILE_MAX_RECT * bin_capacity x 4 B  ((meshlet, tile) pairs)
+ *     direct list  draw_capacity x 4 B      (draws left to the per-meshlet raster) */
 #define LH_CLIP_ENTRY_BYTES 8u
 #define LH_BIG_ENTRY_BYTES 48u
+#define LH_TILE_SIDE 64u       /* pixels per side of a raster tile (one LDS tile of keys) */
+#define LH_TILE_MAX_RECT 16u   /* binned meshlets cover at most this many tiles (larger ones take the
+                                  per-meshlet path) */
+#define LH_BIN_CACHE_BYTES 1280u
 
 typedef struct lh_scene {
     const Instance* instances;              /* [num_instances] */
@@ -127,11 +143,19 @@ typedef struct lh_view {
     const float* tonemap_lut;   /* optional display LUT (tony_mc_mapface, display_transform.comp:5-17):
                                    LH_TONEMAP_LUT_DIMS^3 RGBA float entries, red fastest, indexed by the
                                    stimulus encoded as x / (x + 1); NULL: that encoding is the curve */
+    uint32_t bin_capacity;      /* tiled main-view raster: draws binned into 64x64 screen tiles (the first
+                                   bin_capacity surviving meshlets; later ones take the per-meshlet path);
+                                   0 = per-meshlet raster only.  Keys are identical either way. */
+    uint32_t skybox_size;       /* side of the skybox faces; 0 = constant sky */
+    const float* skybox;        /* optional skybox cubemap (render_geometry.comp:86-101): 6 faces in the order
+                                   +X, -X, +Y, -Y, +Z, -Z, each skybox_size^2 RGBA float texels, rows top
+                                   to bottom; sampled at LOD 0 times 2^3 exposure where nothing is drawn */
 } lh_view;
 #define LH_TONEMAP_LUT_DIMS 48u
 
-/* Bytes of scratch needed for the given queue capacities. */
-uint64_t lh_scratch_bytes(uint32_t big_capacity, uint32_t clip_capacity);
+/* Bytes of scratch the raster needs for the view's sizes and capacities
+ * (width, height, draw_capacity, big_capacity, clip_capacity, bin_capacity). */
+uint64_t lh_scratch_bytes(const lh_view* view);
 
 /* reset_buffers_a (src/shaders/compute/reset_buffers.glsl:9-29) + the
  * attachment clears of src/rendering.cpp:206-211,384: counters and prefix-sum
@@ -271,6 +295,8 @@ typedef struct lh_tuning {
     uint32_t cull_blocks;      /* persistent grid of the meshlet cull */
     uint32_t medium_extent_px; /* up to this extent (small_extent_px..64) the whole wave rasterizes
                                   the triangle in place; larger ones go to the big-triangle queue */
+    uint32_t tile_item_entries; /* tiled main-view raster: binned meshlets per work item (8..512; 0 = 64) */
+    uint32_t tile_blocks;       /* tiled main-view raster: persistent grid of 512-thread blocks (0 = 1024) */
 } lh_tuning;
 /* Sets / reads every class at once (lh_set_tuning) or one class.  Classes by
  * main-view pixels: 0 < 0.6 MP, 1 < 2.5 MP, 2 < 9 MP, 3 larger; class 4 is used

@@ -148,6 +148,7 @@ class LhView(C.Structure):
         ("instance_capacity", C.c_uint32), ("draw_capacity", C.c_uint32), ("width", C.c_uint32),
         ("height", C.c_uint32), ("shadow_size", C.c_uint32), ("big_capacity", C.c_uint32),
         ("clip_capacity", C.c_uint32), ("flags", C.c_uint32), ("history", C.c_uint64), ("tonemap_lut", C.c_uint64),
+        ("bin_capacity", C.c_uint32), ("skybox_size", C.c_uint32), ("skybox", C.c_uint64),
     ]
 
 
@@ -165,6 +166,20 @@ CLIP_ENTRY_BYTES, BIG_ENTRY_BYTES = 8, 48
 LH_OK, LH_ERR_INVALID, LH_ERR_CAPACITY, LH_ERR_HIP, LH_ERR_UNSUPPORTED = 0, 1, 2, 3, 4
 
 
-def scratch_bytes(big_capacity: int, clip_capacity: int) -> int:
-    clip = (clip_capacity * CLIP_ENTRY_BYTES + 255) & ~255
-    return clip + big_capacity * BIG_ENTRY_BYTES
+TILE_SIDE, TILE_MAX_RECT, BIN_CACHE_BYTES = 64, 16, 1280
+CNT_TILE_ITEMS, CNT_TILE_DIRECT = 56, 58
+
+
+def scratch_bytes(width: int, height: int, draw_capacity: int, big_capacity: int, clip_capacity: int,
+                  bin_capacity: int) -> int:
+    """lh_scratch_bytes (include/lh/lh.h): raster queues, then the tiled
+    raster's bins when bin_capacity > 0, each part 256-byte aligned."""
+    a = lambda n: (n + 255) & ~255  # noqa: E731
+    total = a(clip_capacity * CLIP_ENTRY_BYTES) + a(big_capacity * BIG_ENTRY_BYTES)
+    if bin_capacity:
+        tiles = ((width + TILE_SIDE - 1) // TILE_SIDE) * ((height + TILE_SIDE - 1) // TILE_SIDE)
+        entries = TILE_MAX_RECT * bin_capacity
+        items = tiles + entries // 8
+        total += (a(bin_capacity * 16) + a(bin_capacity * BIN_CACHE_BYTES) + 2 * a(tiles * 4) + a(items * 12)
+                  + a(entries * 4) + a(draw_capacity * 4))
+    return total

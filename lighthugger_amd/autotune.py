@@ -28,14 +28,18 @@ SPACE = {
     "big_blocks": [512, 2048],
     "clip_blocks": [64, 256],
     "cull_blocks": [512, 1024, 2048],
+    "tile_item_entries": [32, 64, 128],
+    "tile_blocks": [768, 1024, 2048],
 }
-FIELDS = ("small_extent_px", "meshlet_blocks", "big_blocks", "clip_blocks", "cull_blocks", "medium_extent_px")
+FIELDS = ("small_extent_px", "meshlet_blocks", "big_blocks", "clip_blocks", "cull_blocks", "medium_extent_px",
+          "tile_item_entries", "tile_blocks")
+DEFAULTS = {"tile_item_entries": 64, "tile_blocks": 1024}  # fields absent from older tuning files
 
 
 def _tuning(d: dict):
     from .lib import Tuning
 
-    return Tuning(*[int(d[f]) for f in FIELDS])
+    return Tuning(*[int(d.get(f, DEFAULTS.get(f, 0))) for f in FIELDS])
 
 
 def apply_file(path: Path = TUNING_FILE) -> bool:

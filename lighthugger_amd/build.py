@@ -32,6 +32,7 @@ HIPCC = str(ROCM / "bin" / "hipcc")
 HIP_SOURCES = [
     "cull.hip",
     "raster.hip",
+    "tile.hip",
     "hiz.hip",
     "depth.hip",
     "resolve.hip",

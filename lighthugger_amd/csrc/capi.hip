@@ -57,10 +57,7 @@ hipError_t record_frame(const lh_scene& s, const lh_view& v, hipStream_t st) {
 
 extern "C" {
 
-uint64_t lh_scratch_bytes(uint32_t big_capacity, uint32_t clip_capacity) {
-    uint64_t clip = (uint64_t(clip_capacity) * LH_CLIP_ENTRY_BYTES + 255) & ~uint64_t(255);
-    return clip + uint64_t(big_capacity) * LH_BIG_ENTRY_BYTES;
-}
+uint64_t lh_scratch_bytes(const lh_view* v) { return v ? lhs::scratch_layout(*v).end : 0; }
 
 lh_status lh_view_reset(const lh_scene* s, const lh_view* v, lh_stream stream) {
     if (!lhc::valid_scene(s) || !lhc::valid_view(v)) return LH_ERR_INVALID;
@@ -169,11 +166,14 @@ lh_status lh_view_status(const lh_view* v, uint32_t* host_counters, lh_stream st
 lh_status lh_set_tuning_class(uint32_t cls, const lh_tuning* t) {
     if (cls >= LH_TUNING_CLASSES || !t || t->small_extent_px == 0 || t->small_extent_px > 32 ||
         t->meshlet_blocks == 0 || t->big_blocks == 0 || t->clip_blocks == 0 || t->cull_blocks == 0 ||
-        t->medium_extent_px < t->small_extent_px || t->medium_extent_px > 64)
+        t->medium_extent_px < t->small_extent_px || t->medium_extent_px > 64 ||
+        (t->tile_item_entries && (t->tile_item_entries < 8 || t->tile_item_entries > 512)))
         return LH_ERR_INVALID;
     lhk::Tuning& g = lhk::g_tunings[cls];
     g.raster.small_extent_px = t->small_extent_px;
     g.raster.medium_extent_px = t->medium_extent_px;
+    g.raster.tile_item_entries = t->tile_item_entries ? t->tile_item_entries : 64u;
+    g.raster.tile_blocks = t->tile_blocks ? t->tile_blocks : 1024u;
     g.raster.meshlet_blocks = t->meshlet_blocks;
     g.raster.big_blocks = t->big_blocks;
     g.raster.clip_blocks = t->clip_blocks;
@@ -186,6 +186,8 @@ lh_status lh_get_tuning_class(uint32_t cls, lh_tuning* t) {
     const lhk::Tuning& g = lhk::g_tunings[cls];
     t->small_extent_px = g.raster.small_extent_px;
     t->medium_extent_px = g.raster.medium_extent_px;
+    t->tile_item_entries = g.raster.tile_item_entries;
+    t->tile_blocks = g.raster.tile_blocks;
     t->meshlet_blocks = g.raster.meshlet_blocks;
     t->big_blocks = g.raster.big_blocks;
     t->clip_blocks = g.raster.clip_blocks;

@@ -5,6 +5,7 @@
 #include <cstring>
 
 #include "kernels.h"
+#include "scratch.h"
 
 namespace lhk {
 // Launch tunables per resolution class (lh_tuning_class_of) and for batches.
@@ -39,8 +40,8 @@ inline bool valid_view(const lh_view* v) {
 }
 
 inline bool valid_raster(const lh_view* v) {
-    return v->scratch && v->scratch_bytes >= lh_scratch_bytes(v->big_capacity, v->clip_capacity) &&
-           v->big_capacity > 0 && v->clip_capacity > 0;
+    return v->scratch && v->scratch_bytes >= lhs::scratch_layout(*v).end && v->big_capacity > 0 &&
+           v->clip_capacity > 0 && v->bin_capacity <= v->draw_capacity;
 }
 
 inline bool valid_frame(const lh_scene* s, const lh_view* v) {

@@ -19,6 +19,8 @@ struct RasterTuning {
     uint32_t meshlet_blocks = 8192; // persistent grid of the per-meshlet kernel (64-thread blocks)
     uint32_t big_blocks = 2048;     // persistent grid of the large-triangle kernel (256-thread blocks)
     uint32_t clip_blocks = 256;     // persistent grid of the clip kernel (64-thread blocks)
+    uint32_t tile_item_entries = 64; // tiled main-view raster: binned meshlets per work item (>= 8)
+    uint32_t tile_blocks = 1024;    // tiled main-view raster: persistent grid (512-thread blocks)
 };
 
 struct Tuning {

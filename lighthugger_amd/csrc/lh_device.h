@@ -23,7 +23,7 @@
 #define LH_RASTER_WAVES 8
 #endif
 #ifndef LH_RESOLVE_WAVES
-#define LH_RESOLVE_WAVES 0
+#define LH_RESOLVE_WAVES 4
 #endif
 #define LH_WAVES_ATTR(n) __attribute__((amdgpu_waves_per_eu(n, 8)))
 #if LH_RASTER_WAVES > 0
@@ -126,6 +126,26 @@ LH_DEV uint32_t wave_inclusive_scan(uint32_t v) {
         if (lane >= uint32_t(d)) v += o;
     }
     return v;
+}
+// Inclusive scans across the wave with DPP lane moves (no LDS round trip):
+// shifts of 1, 2, 4, 8 lanes within each row of 16, then row 15 / row 31
+// broadcasts into the rows above.  Lanes a move has no source for keep the
+// identity.
+template <class Op>
+LH_DEV int32_t wave_scan_dpp(int32_t v, int32_t identity, Op op) {
+    v = op(v, __builtin_amdgcn_update_dpp(identity, v, 0x111, 0xf, 0xf, false));  // row_shr:1
+    v = op(v, __builtin_amdgcn_update_dpp(identity, v, 0x112, 0xf, 0xf, false));  // row_shr:2
+    v = op(v, __builtin_amdgcn_update_dpp(identity, v, 0x114, 0xf, 0xf, false));  // row_shr:4
+    v = op(v, __builtin_amdgcn_update_dpp(identity, v, 0x118, 0xf, 0xf, false));  // row_shr:8
+    v = op(v, __builtin_amdgcn_update_dpp(identity, v, 0x142, 0xa, 0xf, false));  // row_bcast:15
+    v = op(v, __builtin_amdgcn_update_dpp(identity, v, 0x143, 0xc, 0xf, false));  // row_bcast:31
+    return v;
+}
+LH_DEV uint32_t wave_scan_add(uint32_t v) {
+    return uint32_t(wave_scan_dpp(int32_t(v), 0, [](int32_t a, int32_t b) { return a + b; }));
+}
+LH_DEV int32_t wave_scan_max(int32_t v) {
+    return wave_scan_dpp(v, INT32_MIN, [](int32_t a, int32_t b) { return max(a, b); });
 }
 LH_DEV uint32_t wave_sum_u32(uint32_t v) {
     for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);

@@ -13,8 +13,8 @@ namespace shading {
 using namespace lhd;
 
 constexpr float kPi = 3.14159265358979323846f;
-// Constant sky radiance: base colour times 2^3 exposure
-// (render_geometry.comp:93-97; the skybox cubemap is absent).
+// Sky radiance without a skybox cubemap (lh_view.skybox NULL): a constant
+// colour times the 2^3 exposure of render_geometry.comp:93-97.
 constexpr float kSkyR = 0.35f * 8.0f, kSkyG = 0.50f * 8.0f, kSkyB = 0.80f * 8.0f;
 
 LH_DEV uint32_t load_index(const MeshInfo& mi, uint32_t i) {
@@ -558,13 +558,93 @@ LH_DEV f3 debug_colour(uint32_t i) {
     }
 }
 
+// Texel (i, j) of skybox face f, clamped to the face's edge.
+LH_DEV f3 sky_texel(const lh_view& v, uint32_t f, int32_t i, int32_t j) {
+    const int32_t n = int32_t(v.skybox_size);
+    i = min(max(i, 0), n - 1);
+    j = min(max(j, 0), n - 1);
+    const float4 t = reinterpret_cast<const float4*>(v.skybox)[(size_t(f) * uint32_t(n) + uint32_t(j)) * uint32_t(n) +
+                                                               uint32_t(i)];
+    return {t.x, t.y, t.z};
+}
+LH_DEV f3 mix3(f3 a, f3 b, float t) { return {mixf(a.x, b.x, t), mixf(a.y, b.y, t), mixf(a.z, b.z, t)}; }
+
+// Skybox radiance of a pixel (render_geometry.comp:86-101): the ray through
+// the pixel corner unprojected by perspective_inverse at z = 0 (the infinitely
+// far plane of the reverse-Z projection), normalized, turned to world space by
+// view_inverse, and looked up at LOD 0 times 2^3.  Face by the major axis
+// (ties go to x, then y) with Vulkan's face table (sc, tc, rc):
+//   +X (-z, -y, x)  -X (z, -y, x)  +Y (x, z, y)  -Y (x, -z, y)  +Z (x, -y, z)  -Z (-x, -y, z),
+// s = (sc / |rc| + 1) / 2, t likewise; bilinear within the face with texel
+// centres at (k + 0.5) / size, clamped to the face's edge (Vulkan filters
+// seamlessly across faces; the two agree except within half a texel of an edge).
+LH_DEV f3 sky_radiance(const lh_view& v, f2 ndc) {
+    if (!v.skybox || v.skybox_size == 0) return {kSkyR, kSkyG, kSkyB};
+    const Uniforms& u = *v.uniforms;
+    const f4 un = mat4_mul_vec4(u.perspective_inverse, f4{ndc.x, ndc.y, 0.0f, 1.0f});
+    const f3 local = normalize3(f3{un.x, un.y, un.z});
+    const f4 r4 = mat4_mul_vec4(u.view_inverse, f4{local.x, local.y, local.z, 0.0f});
+    const float ax = fabsf(r4.x), ay = fabsf(r4.y), az = fabsf(r4.z);
+    uint32_t face;
+    float sc, tc, ma;
+    if (ax >= ay && ax >= az) {
+        face = r4.x >= 0.0f ? 0u : 1u;
+        sc = r4.x >= 0.0f ? -r4.z : r4.z;
+        tc = -r4.y;
+        ma = ax;
+    } else if (ay >= az) {
+        face = r4.y >= 0.0f ? 2u : 3u;
+        sc = r4.x;
+        tc = r4.y >= 0.0f ? r4.z : -r4.z;
+        ma = ay;
+    } else {
+        face = r4.z >= 0.0f ? 4u : 5u;
+        sc = r4.z >= 0.0f ? r4.x : -r4.x;
+        tc = -r4.y;
+        ma = az;
+    }
+    const float n = float(v.skybox_size);
+    const float fs = (sc / ma + 1.0f) * 0.5f, ft = (tc / ma + 1.0f) * 0.5f;
+    const float uu = fs * n - 0.5f, vv = ft * n - 0.5f;
+    const float fu = floorf(uu), fv = floorf(vv);
+    const float a = uu - fu, b = vv - fv;
+    const int32_t i = int32_t(fu), j = int32_t(fv);
+    const f3 c = mix3(mix3(sky_texel(v, face, i, j), sky_texel(v, face, i + 1, j), a),
+                      mix3(sky_texel(v, face, i, j + 1), sky_texel(v, face, i + 1, j + 1), a), b);
+    return mul(c, 8.0f);
+}
+
+LH_DEV float smoothstepf(float e0, float e1, float x) {
+    const float t = clampf((x - e0) / (e1 - e0), 0.0f, 1.0f);
+    return t * t * (3.0f - 2.0f * t);
+}
+// Heat-map colour of t in [0, 1] (temperature(), src/shaders/common/debug.glsl:15-43):
+// the debug colours blended around index 10 t.
+LH_DEV f3 temperature(float t) {
+    const float s = t * 10.0f;
+    const int32_t cur = int32_t(s) <= 9 ? int32_t(s) : 9;
+    const int32_t prv = cur >= 1 ? cur - 1 : 0, nxt = cur < 9 ? cur + 1 : 9;
+    const float blur = 0.8f;
+    const float wc = smoothstepf(float(cur) - blur, float(cur) + blur, s) *
+                     (1.0f - smoothstepf(float(cur + 1) - blur, float(cur + 1) + blur, s));
+    const float wp = 1.0f - smoothstepf(float(cur) - blur, float(cur) + blur, s);
+    const float wn = smoothstepf(float(cur + 1) - blur, float(cur + 1) + blur, s);
+    const f3 a = debug_colour(uint32_t(cur)), b = debug_colour(uint32_t(prv)), c = debug_colour(uint32_t(nxt));
+    return {clampf((wc * a.x + wp * b.x) + wn * c.x, 0.0f, 1.0f), clampf((wc * a.y + wp * b.y) + wn * c.y, 0.0f, 1.0f),
+            clampf((wc * a.z + wp * b.z) + wn * c.z, 0.0f, 1.0f)};
+}
+// UNIFORMS_DEBUG_SHADER_CLOCK: device clock ticks spent on the pixel over
+// 65000, as a heat-map colour (render_geometry.comp:287-301).
+constexpr float kClockHeatScale = 65000.0f;
+
 LH_DEV f3 resolve_pixel(const lh_scene& s, const lh_view& v, uint32_t x, uint32_t y, const lh_resolve_params& prm) {
+    const uint64_t start_time = clock64();
     const Uniforms& u = *v.uniforms;
     const f2 win{float(u.window_size.x), float(u.window_size.y)};
     const f2 ndc{float(x) / win.x * 2.0f - 1.0f, float(y) / win.y * 2.0f - 1.0f};  // pixel corner
     Surface sf;
     const uint64_t key = v.visbuffer[size_t(y) * v.width + x];
-    if (!decode(s, v, key, sf)) return {kSkyR, kSkyG, kSkyB};
+    if (!decode(s, v, key, sf)) return sky_radiance(v, ndc);
     const Instance& in = s.instances[sf.instance];
     const MeshInfo& mi = s.mesh_infos[sf.mesh_info];
     const BaryDeriv bary = calc_full_bary(sf.clip[0], sf.clip[1], sf.clip[2], ndc, win);
@@ -609,8 +689,7 @@ LH_DEV f3 resolve_pixel(const lh_scene& s, const lh_view& v, uint32_t x, uint32_
         nrm = normalize3(f3{(t.x * map.x + b.x * map.y) + nrm.x * map.z, (t.y * map.x + b.y * map.y) + nrm.y * map.z,
                             (t.z * map.x + b.z * map.y) + nrm.z * map.z});
     }
-    // Debug views (render_geometry.comp:268-308); the shader-clock heat map
-    // (UNIFORMS_DEBUG_SHADER_CLOCK) is not reproduced and shades normally.
+    // Debug views (render_geometry.comp:268-308).
     switch (u.debug) {
         case UNIFORMS_DEBUG_CASCADES: return mul(base, debug_colour(cascade));
         case UNIFORMS_DEBUG_TRIANGLE_INDEX: return debug_colour(uint32_t(key & ((1u << LH_KEY_TRI_BITS) - 1)) % 10u);
@@ -626,7 +705,14 @@ LH_DEV f3 resolve_pixel(const lh_scene& s, const lh_view& v, uint32_t x, uint32_
                                 (mi.flags & MESH_INFO_FLAGS_ALPHA_CLIP) != 0),
                            kPi * shadow_sum),
                        param_sun_intensity(prm, u));
-    return add(sun, ambient);
+    const f3 lit = add(sun, ambient);
+    if (u.debug == UNIFORMS_DEBUG_SHADER_CLOCK) {
+        const uint64_t end_time = clock64();
+        // timediff(): at most one wrap of the counter (debug.glsl)
+        const uint64_t dt = end_time >= start_time ? end_time - start_time : uint64_t(~0u) - (start_time - end_time);
+        return temperature(clampf(float(dt) / kClockHeatScale, 0.0f, 1.0f));
+    }
+    return lit;
 }
 
 LH_DEV float srgb_oetf(float x) { return x <= 0.003130f ? x * 12.92f : 1.055f * powf(x, 1.0f / 2.4f) - 0.055f; }

@@ -151,6 +151,7 @@ class ViewConfig:
     ldr: bool = True
     statistics: bool = False  # LH_VIEW_STATISTICS: count covered samples and depth atomics
     hiz: bool = False  # LH_VIEW_HIZ: two-phase occlusion culling of the main view
+    bin_capacity: int = -1  # tiled main-view raster: draws binned per launch; -1 = draw capacity, 0 = off
 
 
 class ViewData:
@@ -160,13 +161,14 @@ class ViewData:
         self.cfg, self.mem = cfg, mem
         icap = cfg.instance_capacity or max(1, scene.scene.num_instances)
         dcap = cfg.draw_capacity or max(1, scene.scene.total_meshlets)
+        bcap = dcap if cfg.bin_capacity < 0 else min(cfg.bin_capacity, dcap)
         W, H, S = cfg.width, cfg.height, cfg.shadow_size
-        self.icap, self.dcap = icap, dcap
+        self.icap, self.dcap, self.bcap = icap, dcap, bcap
         layout = [
             ("uniforms", 560), ("misc", 792), ("prefix", abi.NUM_PASSES * (1 + icap) * 8),
             ("draws", abi.NUM_PASSES * dcap * 8), ("counters", abi.NUM_COUNTERS * 4), ("visbuffer", W * H * 8),
             ("shadow", abi.NUM_CASCADES * S * S * 4), ("radiance", W * H * 16), ("ldr", W * H * 4 if cfg.ldr else 0),
-            ("scratch", abi.scratch_bytes(cfg.big_capacity, cfg.clip_capacity)),
+            ("scratch", abi.scratch_bytes(W, H, dcap, cfg.big_capacity, cfg.clip_capacity, bcap)),
             ("history", (scene.scene.total_meshlets + 31) // 32 * 4),
         ]
         self.offsets, self.sizes, total = {}, {}, 0
@@ -180,7 +182,8 @@ class ViewData:
             a["uniforms"], a["misc"], a["prefix"], a["draws"], a["counters"], a["visbuffer"], a["shadow"],
             a["radiance"], a["ldr"] if cfg.ldr else 0, a["scratch"], self.sizes["scratch"], icap, dcap, W, H, S,
             cfg.big_capacity, cfg.clip_capacity,
-            (abi.VIEW_STATISTICS if cfg.statistics else 0) | (abi.VIEW_HIZ if cfg.hiz else 0), a["history"],
+            (abi.VIEW_STATISTICS if cfg.statistics else 0) | (abi.VIEW_HIZ if cfg.hiz else 0), a["history"], 0,
+            bcap, 0, 0,
         )
         mem.write(self.handle, self.offsets["history"], np.zeros(self.sizes["history"], np.uint8))
         self.uniforms = None

@@ -30,7 +30,8 @@ class Tuning(C.Structure):
     """lh_tuning (include/lh/lh.h)."""
 
     _fields_ = [("small_extent_px", C.c_uint32), ("meshlet_blocks", C.c_uint32), ("big_blocks", C.c_uint32),
-                ("clip_blocks", C.c_uint32), ("cull_blocks", C.c_uint32), ("medium_extent_px", C.c_uint32)]
+                ("clip_blocks", C.c_uint32), ("cull_blocks", C.c_uint32), ("medium_extent_px", C.c_uint32),
+                ("tile_item_entries", C.c_uint32), ("tile_blocks", C.c_uint32)]
 
     def as_dict(self) -> dict:
         return {f: int(getattr(self, f)) for f, _ in self._fields_}
@@ -60,7 +61,7 @@ def lib() -> C.CDLL:
         L.lh_view_status.restype = C.c_int32
         L.lh_view_status.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
         L.lh_scratch_bytes.restype = C.c_uint64
-        L.lh_scratch_bytes.argtypes = [C.c_uint32, C.c_uint32]
+        L.lh_scratch_bytes.argtypes = [C.c_void_p]
         L.lh_set_tuning.restype = C.c_int32
         L.lh_set_tuning.argtypes = [C.POINTER(Tuning)]
         L.lh_get_tuning.restype = C.c_int32

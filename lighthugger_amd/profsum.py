@@ -8,6 +8,7 @@ dispatch) plus the same two runs of the plain copy kernel into per-kernel HBM
 bytes for bench.py's roofline ``traffic`` field.
 
     python -m lighthugger_amd.profsum stats DIR [--title T] [--command C]
+    python -m lighthugger_amd.profsum counters DIR [--match KERNEL]
     python -m lighthugger_amd.profsum traffic --fetch DIR --write DIR \\
         --copy-fetch DIR --copy-write DIR [--copy-bytes N] [--out FILE]
 """
@@ -127,6 +128,19 @@ def print_stats(d: Path, title: str, command: str, limit: int = 25) -> str:
     return "\n".join(lines) + "\n"
 
 
+def print_counters(d: Path, title: str, command: str, match: str = "") -> str:
+    """Mean value per dispatch of every counter collected in `d`, per kernel
+    (kernels whose short name contains `match`)."""
+    lines = [f"# {title}", "", f"command: {command}", ""]
+    for k, cs in sorted(counters(d).items(), key=lambda kv: _short(kv[0])):
+        name = _short(k)
+        if match and match not in name:
+            continue
+        vals = "  ".join(f"{c}={_mean(v):.6g}" for c, v in sorted(cs.items()))
+        lines.append(f"{name[:72]:<72} n={len(next(iter(cs.values())))}  {vals}")
+    return "\n".join(lines) + "\n"
+
+
 def traffic(fetch: Path, write: Path, copy_fetch: Path, copy_write: Path, copy_bytes: int) -> dict:
     """Per frame kernel: HBM bytes per dispatch from FETCH_SIZE / WRITE_SIZE
     (KiB), each scaled by bytes / counter of the plain copy kernel."""
@@ -156,6 +170,11 @@ def main(argv=None) -> None:
     s.add_argument("--title", default="rocprofv3 kernel-trace statistics")
     s.add_argument("--command", default="")
     s.add_argument("--limit", type=int, default=25)
+    c = sub.add_parser("counters")
+    c.add_argument("dir")
+    c.add_argument("--title", default="rocprofv3 counters (mean per dispatch)")
+    c.add_argument("--command", default="")
+    c.add_argument("--match", default="")
     t = sub.add_parser("traffic")
     for k in ("--fetch", "--write", "--copy-fetch", "--copy-write"):
         t.add_argument(k, required=True)
@@ -164,6 +183,8 @@ def main(argv=None) -> None:
     a = ap.parse_args(argv)
     if a.cmd == "stats":
         sys.stdout.write(print_stats(Path(a.dir), a.title, a.command, a.limit))
+    elif a.cmd == "counters":
+        sys.stdout.write(print_counters(Path(a.dir), a.title, a.command, a.match))
     else:
         res = traffic(Path(a.fetch), Path(a.write), Path(a.copy_fetch), Path(a.copy_write), a.copy_bytes)
         text = json.dumps(res, indent=1) + "\n"

@@ -83,6 +83,10 @@ def bind(L: C.CDLL) -> C.CDLL:
     L.orc_sample_texture.argtypes = [C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p]
     L.orc_normal_map.restype = None
     L.orc_normal_map.argtypes = [C.c_void_p] * 5
+    L.orc_sky_radiance.restype = None
+    L.orc_sky_radiance.argtypes = [C.c_void_p] * 3
+    L.orc_temperature.restype = None
+    L.orc_temperature.argtypes = [C.c_float, C.c_void_p]
     return L
 
 

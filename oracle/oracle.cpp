@@ -721,9 +721,59 @@ F3 brdf(F3 v, F3 l, F3 n, float pr, float metallic, F3 base, bool thin) {
     return {std::isnan(irr.x) ? 0.0f : irr.x, std::isnan(irr.y) ? 0.0f : irr.y, std::isnan(irr.z) ? 0.0f : irr.z};
 }
 
-// Constant sky (the reference samples a skybox cubemap that is absent here,
-// render_geometry.comp:86-101): base colour times 2^3 exposure.
+// Sky without a skybox cubemap (lh_view.skybox NULL): a constant colour
+// times the 2^3 exposure of render_geometry.comp:93-97.
 const F3 kSky{0.35f * 8.0f, 0.50f * 8.0f, 0.80f * 8.0f};
+
+// Skybox lookup (render_geometry.comp:86-101): the ray through the pixel
+// corner unprojected by perspective_inverse at z = 0, normalized, rotated to
+// world space by view_inverse; cube face by the major axis (ties: x, then y)
+// with the (sc, tc, rc) table of the Vulkan specification's cube map face
+// selection; s = (sc / |rc| + 1) / 2; bilinear at LOD 0 with the face's edge
+// clamped; times 2^3.
+F3 sky_texel(const lh_view& v, uint32_t f, int32_t i, int32_t j) {
+    const int32_t n = int32_t(v.skybox_size);
+    i = std::min(std::max(i, 0), n - 1);
+    j = std::min(std::max(j, 0), n - 1);
+    const float* t = v.skybox + 4 * ((size_t(f) * uint32_t(n) + uint32_t(j)) * uint32_t(n) + uint32_t(i));
+    return {t[0], t[1], t[2]};
+}
+F3 mix3(F3 a, F3 b, float t) { return {mixf(a.x, b.x, t), mixf(a.y, b.y, t), mixf(a.z, b.z, t)}; }
+F3 sky_radiance(const lh_view& v, F2 ndc) {
+    if (!v.skybox || v.skybox_size == 0) return kSky;
+    const Uniforms& u = *v.uniforms;
+    const F4 un = mat4_mul_vec4(u.perspective_inverse, F4{ndc.x, ndc.y, 0.0f, 1.0f});
+    const F3 local = normalize3(F3{un.x, un.y, un.z});
+    const F4 r = mat4_mul_vec4(u.view_inverse, F4{local.x, local.y, local.z, 0.0f});
+    const float ax = std::fabs(r.x), ay = std::fabs(r.y), az = std::fabs(r.z);
+    uint32_t face;
+    float sc, tc, ma;
+    if (ax >= ay && ax >= az) {
+        face = r.x >= 0.0f ? 0u : 1u;
+        sc = r.x >= 0.0f ? -r.z : r.z;
+        tc = -r.y;
+        ma = ax;
+    } else if (ay >= az) {
+        face = r.y >= 0.0f ? 2u : 3u;
+        sc = r.x;
+        tc = r.y >= 0.0f ? r.z : -r.z;
+        ma = ay;
+    } else {
+        face = r.z >= 0.0f ? 4u : 5u;
+        sc = r.z >= 0.0f ? r.x : -r.x;
+        tc = -r.y;
+        ma = az;
+    }
+    const float n = float(v.skybox_size);
+    const float fs = (sc / ma + 1.0f) * 0.5f, ft = (tc / ma + 1.0f) * 0.5f;
+    const float uu = fs * n - 0.5f, vv = ft * n - 0.5f;
+    const float fu = std::floor(uu), fv = std::floor(vv);
+    const float a = uu - fu, b = vv - fv;
+    const int32_t i = int32_t(fu), j = int32_t(fv);
+    const F3 c = mix3(mix3(sky_texel(v, face, i, j), sky_texel(v, face, i + 1, j), a),
+                      mix3(sky_texel(v, face, i, j + 1), sky_texel(v, face, i + 1, j + 1), a), b);
+    return mul(c, 8.0f);
+}
 
 // Cascade selection (render_geometry.comp:183-196; the loop may end at 4,
 // which the array layer clamps to 3) and 3x3 PCF (:206-226).
@@ -778,6 +828,26 @@ F3 debug_colour(uint32_t i) {
     return {c[0] / 255.0f, c[1] / 255.0f, c[2] / 255.0f};
 }
 
+// GLSL smoothstep and temperature() (src/shaders/common/debug.glsl:15-43),
+// the heat map of the shader-clock debug view.
+float smoothstepf(float e0, float e1, float x) {
+    const float t = clampf((x - e0) / (e1 - e0), 0.0f, 1.0f);
+    return t * t * (3.0f - 2.0f * t);
+}
+F3 temperature(float t) {
+    const float s = t * 10.0f;
+    const int32_t cur = int32_t(s) <= 9 ? int32_t(s) : 9;
+    const int32_t prv = cur >= 1 ? cur - 1 : 0, nxt = cur < 9 ? cur + 1 : 9;
+    const float blur = 0.8f;
+    const float wc = smoothstepf(float(cur) - blur, float(cur) + blur, s) *
+                     (1.0f - smoothstepf(float(cur + 1) - blur, float(cur + 1) + blur, s));
+    const float wp = 1.0f - smoothstepf(float(cur) - blur, float(cur) + blur, s);
+    const float wn = smoothstepf(float(cur + 1) - blur, float(cur + 1) + blur, s);
+    const F3 a = debug_colour(uint32_t(cur)), b = debug_colour(uint32_t(prv)), c = debug_colour(uint32_t(nxt));
+    return {clampf((wc * a.x + wp * b.x) + wn * c.x, 0.0f, 1.0f), clampf((wc * a.y + wp * b.y) + wn * c.y, 0.0f, 1.0f),
+            clampf((wc * a.z + wp * b.z) + wn * c.z, 0.0f, 1.0f)};
+}
+
 // render_geometry main() (src/shaders/render_geometry.comp:70-266) for one pixel.
 F3 resolve_pixel(const lh_scene& sc, const lh_view& v, uint32_t x, uint32_t y) {
     const Uniforms& u = *v.uniforms;
@@ -785,7 +855,7 @@ F3 resolve_pixel(const lh_scene& sc, const lh_view& v, uint32_t x, uint32_t y) {
     F2 win{float(u.window_size.x), float(u.window_size.y)};
     F2 ndc{float(x) / win.x * 2.0f - 1.0f, float(y) / win.y * 2.0f - 1.0f};  // pixel corner (:82-84)
     uint64_t key = v.visbuffer[size_t(y) * v.width + x];
-    if (~uint32_t(key >> 32) == 0u) return kSky;  // depth 0.0
+    if (~uint32_t(key >> 32) == 0u) return sky_radiance(v, ndc);  // depth 0.0
     uint32_t mid = uint32_t(key >> LH_KEY_TRI_BITS) & (LH_MAX_SCENE_MESHLETS - 1);
     uint32_t tri = uint32_t(key) & ((1u << LH_KEY_TRI_BITS) - 1);
     const MeshletReference& ref = sc.meshlet_table[mid];
@@ -829,7 +899,8 @@ F3 resolve_pixel(const lh_scene& sc, const lh_view& v, uint32_t x, uint32_t y) {
     float metallic = mr.z * fac.metallic_factor;
     if (mi.normal_texture_index != UNUSED_TEXTURE_INDEX)
         nrm = normal_map(nrm, world, uv, sample_texture(sc, mi.normal_texture_index, uv));
-    // Debug views (:268-308); the shader-clock heat map is not reproduced.
+    // Debug views (:268-308); the shader-clock heat map has no clock here and
+    // shades normally (the kernel's heat map is checked against temperature()).
     if (u.debug == UNIFORMS_DEBUG_CASCADES) return mul(base, debug_colour(cascade));
     if (u.debug == UNIFORMS_DEBUG_TRIANGLE_INDEX) return debug_colour(tri % 10u);
     if (u.debug == UNIFORMS_DEBUG_INSTANCE_INDEX) return debug_colour(uint32_t(ref.meshlet_index) % 10u);
@@ -1342,6 +1413,16 @@ void orc_brdf(const float* v, const float* l, const float* n, float r, float m, 
     o[2] = x.z;
 }
 float orc_srgb_oetf(float x) { return srgb_oetf(x); }
+
+void orc_temperature(float t, float* rgb) {
+    const F3 c = temperature(t);
+    rgb[0] = c.x, rgb[1] = c.y, rgb[2] = c.z;
+}
+
+void orc_sky_radiance(const lh_view* v, const float* ndc, float* rgb) {
+    const F3 c = sky_radiance(*v, F2{ndc[0], ndc[1]});
+    rgb[0] = c.x, rgb[1] = c.y, rgb[2] = c.z;
+}
 
 void orc_bc7_block(const uint8_t* block, uint8_t* rgba64) {
     orc_texel::RGBA8 t[16];

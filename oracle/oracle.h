@@ -92,6 +92,13 @@ float orc_srgb_to_linear(uint32_t v);
  *                      (:25-38): normal[3], world9 = {p, dp/dx, dp/dy},
  *                      uv6 as above, texel[4] the sampled RGBA in [0, 1]. */
 float orc_shadow_tap(const lh_view* view, uint32_t layer, const float* uv, float ref);
+/* Sky radiance of the pixel corner ndc[2] (render_geometry.comp:86-101):
+ * lh_view.skybox looked up along the unprojected view ray, times 2^3; the
+ * constant sky without a skybox. */
+void orc_sky_radiance(const lh_view* view, const float* ndc, float* rgb3);
+/* Heat-map colour temperature(t) of the shader-clock debug view
+ * (src/shaders/common/debug.glsl:15-43). */
+void orc_temperature(float t, float* rgb3);
 float orc_shadow_factor(const lh_view* view, const MeshInfo* mesh_info, const float* world, uint32_t* cascade);
 void orc_texture_lod(const LhTexture* texture, const float* uv6, uint32_t* l0, uint32_t* l1, float* f);
 void orc_bilinear(const LhTexture* texture, uint32_t level, const float* uv, float* rgba);

@@ -32,7 +32,7 @@ int main(void) {
   O(MeshInfo, base_color_factor) O(Instance, normal_transform) O(Meshlet, index_count)
   O(Uniforms, debug_shadowmaps) O(MiscStorage, max_depth) O(LhTexture, level_offset)
   O(lh_scene, num_instances) O(lh_scene, num_textures) O(lh_scene, flags) O(lh_view, scratch_bytes) O(lh_view, instance_capacity)
-  O(lh_view, flags) O(lh_view, history) O(lh_camera, shadow_cam_distance) O(lh_scene_arrays, unique_triangles)
+  O(lh_view, flags) O(lh_view, history) O(lh_view, bin_capacity) O(lh_camera, shadow_cam_distance) O(lh_scene_arrays, unique_triangles)
   return 0;
 }
 """
@@ -79,6 +79,7 @@ def test_field_offsets(probe):
     assert probe["lh_view.instance_capacity"] == abi.LhView.instance_capacity.offset
     assert probe["lh_view.flags"] == abi.LhView.flags.offset
     assert probe["lh_view.history"] == abi.LhView.history.offset
+    assert probe["lh_view.bin_capacity"] == abi.LhView.bin_capacity.offset
     assert probe["lh_camera.shadow_cam_distance"] == abi.Camera.shadow_cam_distance.offset
     assert probe["lh_scene_arrays.unique_triangles"] == abi.SceneArrays.unique_triangles.offset
 
@@ -102,7 +103,12 @@ def test_host_only_entry_points_without_gpu():
 
     L = lib()
     assert L.lh_version().startswith(b"lighthugger_amd")
-    assert L.lh_scratch_bytes(10, 100) == abi.scratch_bytes(10, 100) == 1024 + 480
+    v = abi.LhView()
+    v.big_capacity, v.clip_capacity = 10, 100
+    assert L.lh_scratch_bytes(C.byref(v)) == abi.scratch_bytes(0, 0, 0, 10, 100, 0) == 1024 + 512
+    v.width, v.height, v.draw_capacity, v.bin_capacity = 1920, 1080, 5000, 3000
+    assert L.lh_scratch_bytes(C.byref(v)) == abi.scratch_bytes(1920, 1080, 5000, 10, 100, 3000)
+    assert abi.scratch_bytes(1920, 1080, 5000, 10, 100, 3000) > 3000 * abi.BIN_CACHE_BYTES
     t = get_tuning()
     assert 1 <= t.small_extent_px <= 32
     bad = Tuning(0, 1, 1, 1, 1, 8)

@@ -140,6 +140,55 @@ def test_tonemap_lut_matches_oracle():
     assert (gpu["ldr"] != orc["ldr"]).mean() < 1e-3
 
 
+def test_skybox_and_shader_clock_view():
+    """A supplied skybox cubemap (lh_view.skybox, random faces): every pixel
+    matches the oracle at the radiance bar.  The shader-clock debug view
+    (timed on the device clock, so not comparable bit for bit) colours every
+    shaded pixel with a point of the heat-map curve temperature(t)."""
+    import torch
+
+    from lighthugger_amd.lib import Renderer
+    from oracle import lib as oracle_lib
+    from oracle.compare import oracle_view
+
+    scene = HostScene.preset(abi.PRESET_SMALL)
+    cam = preset_camera(abi.PRESET_SMALL)
+    cfg = ViewConfig(cam.width, cam.height, 512)
+    faces = np.random.default_rng(4).random((6, 96, 96, 4), dtype=np.float32)
+    dev = torch.from_numpy(faces).cuda()
+    r = Renderer(scene, cfg)
+    r.set_camera(cam)
+    r.view.c.skybox, r.view.c.skybox_size = dev.data_ptr(), 96
+    r.frame()
+    torch.cuda.synchronize()
+    gpu = snapshot(r.view)
+    sd, vd = oracle_view(scene, cfg, make_uniforms(cam, scene.num_instances))
+    vd.c.skybox, vd.c.skybox_size = faces.ctypes.data, 96
+    oracle_lib().orc_frame(sd.address, vd.address)
+    orc = snapshot(vd)
+    rep = compare_frames(gpu, orc, scene)
+    sky = gpu["visbuffer"] == np.uint64(abi.LH_VISBUFFER_CLEAR)
+    print(rep, "sky pixels", int(sky.sum()), flush=True)
+    assert sky.sum() > 1000 and np.unique(gpu["radiance"][sky][:, 0]).size > 100
+    # shader-clock view
+    u = make_uniforms(cam, scene.num_instances)
+    u["debug"] = abi.UNIFORMS_DEBUG_SHADER_CLOCK
+    r.view.set_uniforms(u, r.scene)
+    r.frame()
+    torch.cuda.synchronize()
+    rad = r.view.radiance()[..., :3]
+    cov = ~sky
+    curve = np.zeros((8001, 3), np.float32)
+    for k, t in enumerate(np.linspace(0.0, 1.0, 8001)):
+        oracle_lib().orc_temperature(float(t), curve[k].ctypes.data)
+    got = np.unique(rad[cov], axis=0)
+    d = np.concatenate([np.min(np.linalg.norm(got[i: i + 1024, None, :] - curve[None, :, :], axis=2), axis=1)
+                        for i in range(0, len(got), 1024)])
+    print("clock view:", len(got), "distinct colours, max distance to the heat-map curve", float(d.max()), flush=True)
+    assert d.max() < 2e-3
+    assert np.allclose(rad[sky], gpu["radiance"][sky][:, :3])  # the sky is unchanged
+
+
 @pytest.mark.parametrize("texture_format", [abi.TEXTURE_BC1, abi.TEXTURE_BC1 | abi.TEXTURE_SRGB,
                                             abi.TEXTURE_BC7 | abi.TEXTURE_SRGB, abi.TEXTURE_RGBA8 | abi.TEXTURE_SRGB],
                          ids=["bc1", "bc1_srgb", "bc7_srgb", "rgba8_srgb"])
@@ -225,8 +274,8 @@ def test_output_independent_of_tuning():
     saved = get_tuning()
     ref = None
     try:
-        for t in (Tuning(1, 8192, 2048, 256, 1024, 1), Tuning(2, 4096, 512, 64, 512, 16),
-                  Tuning(8, 16384, 2048, 256, 2048, 64), Tuning(32, 1024, 64, 16, 128, 64)):
+        for t in (Tuning(1, 8192, 2048, 256, 1024, 1, 8, 256), Tuning(2, 4096, 512, 64, 512, 16, 512, 2048),
+                  Tuning(8, 16384, 2048, 256, 2048, 64, 32, 64), Tuning(32, 1024, 64, 16, 128, 64, 0, 0)):
             set_tuning(t)
             r.frame()
             torch.cuda.synchronize()
@@ -238,6 +287,43 @@ def test_output_independent_of_tuning():
                 assert np.array_equal(ref[k], s[k]), (t.as_dict(), k)
     finally:
         set_tuning(saved)
+
+
+@pytest.mark.parametrize("hiz", [False, True])
+def test_tiled_raster_equals_per_meshlet_raster(hiz):
+    """The main view through the LDS-binned tile raster (every surviving
+    meshlet binned), partly binned (a bin capacity of 2,000 draws: the rest
+    take the per-meshlet raster) and not at all (bin_capacity 0): identical
+    keys, shadows and radiance, and the oracle's."""
+    import torch
+
+    from lighthugger_amd.lib import Renderer
+
+    scene = HostScene.preset(abi.PRESET_LARGE)
+    cam = preset_camera(abi.PRESET_LARGE)
+    snaps = []
+    for bins in (-1, 2000, 0):
+        cfg = ViewConfig(cam.width, cam.height, 2048, big_capacity=1 << 21, bin_capacity=bins, hiz=hiz)
+        r = Renderer(scene, cfg)
+        r.set_camera(cam)
+        r.frame()
+        r.frame()  # with Hi-Z the second frame uses the first one's visibility history
+        torch.cuda.synchronize()
+        st = r.status()
+        print(f"bins {bins}: tile items {int(st[abi.CNT_TILE_ITEMS])}, direct draws {int(st[abi.CNT_TILE_DIRECT])}",
+              flush=True)
+        assert st[abi.CNT_OVERFLOW] == 0
+        if bins == 2000:
+            assert st[abi.CNT_TILE_DIRECT] > 0
+        snaps.append(snapshot(r.view))
+        del r
+    for s in snaps[1:]:
+        for k in ("visbuffer", "shadow", "radiance", "ldr"):
+            assert np.array_equal(snaps[0][k], s[k]), k
+    if not hiz:
+        orc = oracle_frame(scene, ViewConfig(cam.width, cam.height, 2048), make_uniforms(cam, scene.num_instances),
+                           threads=8)
+        print(compare_frames(snaps[0], orc, scene), flush=True)
 
 
 def test_capacity_overflow_reported_on_device():
