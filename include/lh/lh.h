@@ -247,6 +247,22 @@ lh_status lh_resolve_backward(const lh_scene* scene, const lh_view* view, const 
 /* The whole frame in the reference's pass order (src/rendering.cpp:79-502). */
 lh_status lh_frame(const lh_scene* scene, const lh_view* view, lh_stream stream);
 
+/* lh_frame with the raster scratch in a device array that the library owns
+ * instead of view->scratch.  No counterpart in the reference: this entry
+ * point is for callers that cannot allocate device memory (code under stream
+ * capture, tight serving loops, harnesses that forbid hipMalloc); it
+ * allocates nothing.  view->scratch and view->scratch_bytes are ignored;
+ * lh_scratch_bytes(view) must not exceed LH_STATIC_SCRATCH_BYTES (else
+ * LH_ERR_CAPACITY).  The array is part of the library's device code, one per
+ * device, held until the process ends; its address is looked up once per
+ * device on the first call made with that device current, and `stream` must
+ * belong to the current device.  Every call on one device shares the array
+ * (it keeps the last call's contents; the frame writes every word it reads
+ * first): calls that overlap on one device, from two streams or threads,
+ * race. */
+#define LH_STATIC_SCRATCH_BYTES (64ull << 20)
+lh_status lh_frame_noalloc(const lh_scene* scene, const lh_view* view, lh_stream stream);
+
 /* lh_frame with per-kernel device times: an event is recorded on `stream`
  * before every kernel launch, the call synchronizes on the last one and
  * writes one entry per kernel (in launch order).  No counterpart in the

@@ -167,6 +167,7 @@ LH_OK, LH_ERR_INVALID, LH_ERR_CAPACITY, LH_ERR_HIP, LH_ERR_UNSUPPORTED = 0, 1, 2
 
 
 TILE_SIDE, TILE_MAX_RECT, BIN_CACHE_BYTES = 64, 16, 1280
+STATIC_SCRATCH_BYTES = 64 << 20  # LH_STATIC_SCRATCH_BYTES: the scratch array of lh_frame_noalloc
 CNT_TILE_ITEMS, CNT_TILE_DIRECT = 56, 58
 
 

@@ -39,6 +39,7 @@ HIP_SOURCES = [
     "backward.hip",
     "capi.hip",
     "batch.hip",
+    "noalloc.hip",
 ]
 
 # Device code: gfx950 only; no FMA contraction; IEEE div/sqrt; packed fp32

@@ -67,6 +67,8 @@ def lib() -> C.CDLL:
         L.lh_get_tuning.restype = C.c_int32
         L.lh_get_tuning.argtypes = [C.POINTER(Tuning)]
         L.lh_version.restype = C.c_char_p
+        L.lh_frame_noalloc.restype = C.c_int32
+        L.lh_frame_noalloc.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
         L.lh_frame_profiled.restype = C.c_int32
         L.lh_frame_profiled.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(KernelTime), C.c_uint32,
                                         C.POINTER(C.c_uint32)]
