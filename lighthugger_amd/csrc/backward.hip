@@ -23,7 +23,8 @@
 // Work split: a persistent grid of one-wave blocks walks the 8x8 pixel tiles.
 // Per tile the gradients of the pixels' vertices are merged in an LDS hash
 // table and added to the gradient arena once per distinct vec3; material-factor
-// gradients are summed over the wave per distinct mesh info and added once;
+// gradients are summed over the wave per distinct mesh info and kept per block
+// in registers (one lane per mesh info) until the block ends;
 // the six light gradients stay in registers and every block writes one
 // partial, which a second one-block kernel sums in a fixed order.
 #include "kernels.h"
@@ -196,56 +197,97 @@ struct LightGrad {
     d3 sun_dir, sun_intensity;
 };
 
-// Vertex gradients of one tile are merged in an LDS hash table keyed by the
-// gradient-arena word of the vec3 (positions and normals alike) and flushed
-// with one atomic per distinct word: neighbouring pixels share vertices, and
-// same-address atomics from one wave serialize.  A tile inserts at most
-// 64 pixels x 6 vec3 = 384 keys: the 1024-slot table stays below 38% load
-// (short probe chains) while two blocks per SIMD still fit in LDS, and the
-// slots taken are listed so the flush visits only those.
-constexpr uint32_t kHashSlots = 1024;
-constexpr uint32_t kHashKeysMax = 384;
-constexpr uint32_t kHashEmpty = 0xFFFFFFFFu;
+// Vertex gradients of one tile are merged in an LDS hash table before they
+// reach the gradient arena: neighbouring pixels share vertices, and
+// same-address atomics from one wave serialize.  One entry per vertex holds
+// its position and normal gradients (six floats), keyed by the arena words of
+// both vec3s (position word | normal word << 32; ~0u for a vec3 outside the
+// arena), so a mesh info whose normals differ from another's over the same
+// positions gets entries of its own.  A tile claims at most 64 pixels x
+// 3 vertices = 192 entries: the 512-slot table stays below 38% load (short
+// probe chains) in 16 KB, so two one-wave blocks per SIMD fit in LDS.  Claimed
+// slots are listed (one LDS atomic per wave and probe round) so the flush
+// visits only those.
+constexpr uint32_t kHashSlots = 512;
+constexpr uint32_t kHashKeysMax = 192;
+constexpr uint64_t kHashEmpty = ~0ull;
+constexpr uint32_t kNoWord = ~0u;
 struct VertexHash {
-    uint32_t* keys;   // [kHashSlots]
-    float* vals;      // [kHashSlots * 3]
-    uint32_t* used;   // [kHashKeysMax] slots taken since the last flush
+    uint64_t* keys;   // [kHashSlots]
+    float* vals;      // [kHashSlots * 6]: position gradient, normal gradient
+    uint32_t* used;   // [kHashKeysMax] slots claimed since the last flush
     uint32_t* count;  // number of them
 };
 
-LH_DEV void add_geometry(const lh_resolve_grads& gr, const VertexHash& h, uint64_t addr, f3 g) {
-    if (!gr.geometry || addr < gr.geometry_base) return;
-    const uint64_t k64 = (addr - gr.geometry_base) / 4;
-    if (k64 + 3 > gr.geometry_bytes / 4 || k64 >= kHashEmpty) return;
-    const uint32_t k = uint32_t(k64);
-    uint32_t slot = (k * 2654435761u) >> 22;  // 10 bits: kHashSlots == 1024
-    while (true) {
-        const uint32_t old = atomicCAS(h.keys + slot, kHashEmpty, k);
-        if (old == kHashEmpty) {
-            const uint32_t pos = atomicAdd(h.count, 1u);
-            if (pos < kHashKeysMax) h.used[pos] = slot;
-            break;
-        }
-        if (old == k) break;
-        slot = (slot + 1) & (kHashSlots - 1);
+// Word index of a vec3 in the gradient arena, or kNoWord outside it.
+LH_DEV uint32_t arena_word(const lh_resolve_grads& gr, uint64_t addr) {
+    if (!gr.geometry || addr < gr.geometry_base) return kNoWord;
+    const uint64_t k = (addr - gr.geometry_base) / 4;
+    return (k + 3 > gr.geometry_bytes / 4 || k >= kNoWord) ? kNoWord : uint32_t(k);
+}
+
+// Claims the entries of the three vertices of each active lane's pixel, all
+// three probes in flight together, and returns their slots (kHashSlots for a
+// vertex with neither vec3 in the arena).
+LH_DEV void claim_vertices(const VertexHash& h, const uint64_t key[3], uint32_t slot[3]) {
+    bool done[3];
+    for (int j = 0; j < 3; j++) {
+        done[j] = key[j] == kHashEmpty;
+        const uint32_t mix = uint32_t(key[j]) * 2654435761u ^ uint32_t(key[j] >> 32) * 0x85EBCA6Bu;
+        slot[j] = done[j] ? kHashSlots : mix >> 23;  // 9 bits: kHashSlots == 512
     }
-    atomicAdd(h.vals + 3 * slot, g.x);
-    atomicAdd(h.vals + 3 * slot + 1, g.y);
-    atomicAdd(h.vals + 3 * slot + 2, g.z);
+    while (!(done[0] && done[1] && done[2])) {
+        uint64_t old[3];
+        for (int j = 0; j < 3; j++)
+            if (!done[j]) old[j] = atomicCAS(reinterpret_cast<unsigned long long*>(h.keys) + slot[j],
+                                             kHashEmpty, key[j]);
+        for (int j = 0; j < 3; j++) {
+            const bool fresh = !done[j] && old[j] == kHashEmpty;
+            const uint64_t m = ballot(fresh);
+            if (m) {
+                uint32_t base = 0;
+                if (lane_id() == first_lane(m)) base = atomicAdd(h.count, uint32_t(__popcll(m)));
+                base = __builtin_amdgcn_readlane(base, int(first_lane(m)));
+                if (fresh) h.used[base + lanes_below(m)] = slot[j];
+            }
+            if (!done[j]) {
+                if (fresh || old[j] == key[j]) done[j] = true;
+                else slot[j] = (slot[j] + 1) & (kHashSlots - 1);
+            }
+        }
+    }
+}
+
+LH_DEV void add_vertex(const VertexHash& h, uint32_t slot, uint32_t part, f3 g) {
+    if (slot >= kHashSlots) return;
+    float* val = h.vals + 6 * slot + part;
+    atomicAdd(val, g.x);
+    atomicAdd(val + 1, g.y);
+    atomicAdd(val + 2, g.z);
 }
 
 // Called by the whole wave after a tile: adds every merged vec3 to the arena
 // and empties the table.
 LH_DEV void flush_geometry(const lh_resolve_grads& gr, const VertexHash& h) {
     __syncthreads();
-    const uint32_t n = min(*h.count, kHashKeysMax);
+    const uint32_t n = *h.count;
     for (uint32_t i = threadIdx.x; i < n; i += 64) {
-        const uint32_t slot = h.used[i], k = h.keys[slot];
-        atomicAdd(gr.geometry + k, h.vals[3 * slot]);
-        atomicAdd(gr.geometry + k + 1, h.vals[3 * slot + 1]);
-        atomicAdd(gr.geometry + k + 2, h.vals[3 * slot + 2]);
+        const uint32_t slot = h.used[i];
+        const uint64_t key = h.keys[slot];
+        float* val = h.vals + 6 * slot;
+        const uint32_t kp = uint32_t(key), kn = uint32_t(key >> 32);
+        if (kp != kNoWord) {
+            atomicAdd(gr.geometry + kp, val[0]);
+            atomicAdd(gr.geometry + kp + 1, val[1]);
+            atomicAdd(gr.geometry + kp + 2, val[2]);
+        }
+        if (kn != kNoWord) {
+            atomicAdd(gr.geometry + kn, val[3]);
+            atomicAdd(gr.geometry + kn + 1, val[4]);
+            atomicAdd(gr.geometry + kn + 2, val[5]);
+        }
         h.keys[slot] = kHashEmpty;
-        h.vals[3 * slot] = h.vals[3 * slot + 1] = h.vals[3 * slot + 2] = 0.0f;
+        for (int c = 0; c < 6; c++) val[c] = 0.0f;
     }
     __syncthreads();
     if (threadIdx.x == 0) *h.count = 0;
@@ -428,10 +470,11 @@ LH_DEV void pixel_bwd(const lh_scene& s, const lh_view& v, const lh_resolve_para
     // a common shift of the lambda gradients cancels and the relative form
     // keeps their differences exact
     double glam[3];
+    f3 gnraw[3];
     for (int j = 0; j < 3; j++) {
         glam[j] = double(dot3(gworld, dpos[j])) + double(dot3(gnI, sub(nn[j], nn[0]))) +
                   double(guv.x * duv[j].x + guv.y * duv[j].y);
-        add_geometry(gr, hash, mi.normals + 12ull * sf.vi[j], normalize_bwd(nraw[j], mul(gnI, lamf[j])));
+        gnraw[j] = normalize_bwd(nraw[j], mul(gnI, lamf[j]));
     }
     const double lg_dot = (glam[0] * lam[0] + glam[1] * lam[1]) + glam[2] * lam[2];
     d3 gc[3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
@@ -441,6 +484,15 @@ LH_DEV void pixel_bwd(const lh_scene& s, const lh_view& v, const lh_resolve_para
         gc[a] = gc[a] + cross(c[b], q) * gD;
         gc[b] = gc[b] + cross(q, c[a]) * gD;
     }
+    uint32_t slot[3];
+    {
+        uint64_t key[3];
+        for (int j = 0; j < 3; j++)
+            key[j] = uint64_t(arena_word(gr, mi.positions + 12ull * sf.vi[j])) |
+                     uint64_t(arena_word(gr, mi.normals + 12ull * sf.vi[j])) << 32;
+        claim_vertices(hash, key, slot);
+    }
+    for (int j = 0; j < 3; j++) add_vertex(hash, slot[j], 3, gnraw[j]);
     const LhMat4& T = in.transform;
     for (int j = 0; j < 3; j++) {
         d3 gpos = d3{gworld.x, gworld.y, gworld.z} * lam[j];
@@ -453,7 +505,7 @@ LH_DEV void pixel_bwd(const lh_scene& s, const lh_view& v, const lh_resolve_para
             if (k == 2) gpos.z += d;
         }
         const d3 gL{dot(col3(T.c[0]), gpos), dot(col3(T.c[1]), gpos), dot(col3(T.c[2]), gpos)};
-        add_geometry(gr, hash, mi.positions + 12ull * sf.vi[j], f3{float(gL.x), float(gL.y), float(gL.z)});
+        add_vertex(hash, slot[j], 0, f3{float(gL.x), float(gL.y), float(gL.z)});
     }
 }
 
@@ -472,20 +524,35 @@ template <bool kRecorded>
 __global__ void __launch_bounds__(64) LH_BACKWARD_OCCUPANCY k_resolve_backward(lh_scene s, OneView vs, lh_resolve_params prm,
                                                          const float* grad_radiance, lh_resolve_grads gr,
                                                          double* partials) {
-    __shared__ uint32_t hkeys[kHashSlots];
-    __shared__ float hvals[kHashSlots * 3];
+    __shared__ uint64_t hkeys[kHashSlots];
+    __shared__ float hvals[kHashSlots * 6];
     __shared__ uint32_t hused[kHashKeysMax];
     __shared__ uint32_t hcount;
     const VertexHash hash{hkeys, hvals, hused, &hcount};
     for (uint32_t i = threadIdx.x; i < kHashSlots; i += 64) {
         hkeys[i] = kHashEmpty;
-        hvals[3 * i] = hvals[3 * i + 1] = hvals[3 * i + 2] = 0.0f;
+        for (int c = 0; c < 6; c++) hvals[6 * i + c] = 0.0f;
     }
     if (threadIdx.x == 0) hcount = 0;
     __syncthreads();
     const lh_view& v = vs.v;
     const uint32_t tiles_x = (v.width + 7) / 8, tiles = tiles_x * ((v.height + 7) / 8);
     LightGrad lg{{0, 0, 0}, {0, 0, 0}};
+    // Material-factor gradients: lane i keeps the sums of the i-th distinct
+    // mesh info the block met and adds them to the outputs once, at the end.
+    // The few mesh infos that cover most of the view would otherwise take
+    // five same-address atomics per tile, which serialize in the L2.
+    uint32_t mat_key = ~0u, mat_used = 0;  // mat_used: wave-uniform
+    float mat_val[5] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+    auto add_material = [&](uint32_t mi, const float* val) {
+        if (gr.base_color_factor) {
+            atomicAdd(gr.base_color_factor + 3 * mi, val[0]);
+            atomicAdd(gr.base_color_factor + 3 * mi + 1, val[1]);
+            atomicAdd(gr.base_color_factor + 3 * mi + 2, val[2]);
+        }
+        if (gr.roughness_factor) atomicAdd(gr.roughness_factor + mi, val[3]);
+        if (gr.metallic_factor) atomicAdd(gr.metallic_factor + mi, val[4]);
+    };
     for (uint32_t t = blockIdx.x; t < tiles; t += gridDim.x) {
         const uint32_t x = (t % tiles_x) * 8 + (threadIdx.x & 7), y = (t / tiles_x) * 8 + (threadIdx.x >> 3);
         uint32_t mesh_info = ~0u;
@@ -506,18 +573,25 @@ __global__ void __launch_bounds__(64) LH_BACKWARD_OCCUPANCY k_resolve_backward(l
                 for (int d = 32; d >= 1; d >>= 1) sum += __shfl_xor(sum, d, 64);
                 r[i] = sum;
             }
-            if (lane_id() == first_lane(pending)) {
-                if (gr.base_color_factor) {
-                    atomicAdd(gr.base_color_factor + 3 * lead, float(r[0]));
-                    atomicAdd(gr.base_color_factor + 3 * lead + 1, float(r[1]));
-                    atomicAdd(gr.base_color_factor + 3 * lead + 2, float(r[2]));
-                }
-                if (gr.roughness_factor) atomicAdd(gr.roughness_factor + lead, float(r[3]));
-                if (gr.metallic_factor) atomicAdd(gr.metallic_factor + lead, float(r[4]));
+            // every lane holds the wave sums; pick the lane that keeps lead
+            const uint64_t hit = ballot(mat_key == lead);
+            uint32_t owner = 64;  // none: the table is full, add directly
+            if (hit) {
+                owner = first_lane(hit);
+            } else if (mat_used < 64) {
+                owner = mat_used++;
+                if (lane_id() == owner) mat_key = lead;
+            }
+            if (lane_id() == owner) {
+                for (int i = 0; i < 5; i++) mat_val[i] += float(r[i]);
+            } else if (owner == 64 && lane_id() == first_lane(pending)) {
+                const float val[5] = {float(r[0]), float(r[1]), float(r[2]), float(r[3]), float(r[4])};
+                add_material(lead, val);
             }
             pending &= ~ballot(mine);
         }
     }
+    if (mat_key != ~0u) add_material(mat_key, mat_val);
     double r[6] = {lg.sun_dir.x, lg.sun_dir.y, lg.sun_dir.z, lg.sun_intensity.x, lg.sun_intensity.y,
                    lg.sun_intensity.z};
     for (int i = 0; i < 6; i++) {
