@@ -64,11 +64,19 @@ LH_DEV float pow4f(float x) {
 LH_DEV float pow5f(float x) { return pow4f(x) * x; }
 LH_DEV f3 madd(f3 a, f3 b, float s) { return add(a, mul(b, s)); }
 
+// Reciprocal and reciprocal square root to about 1 ulp: the FP32 part of the
+// adjoint is held to a relative 1e-4, far wider than what the IEEE-rounded
+// division and square root it replaces would buy, at a tenth of the
+// instructions.
+LH_DEV float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+LH_DEV float rsqrt(float x) { return __builtin_amdgcn_rsqf(x); }
+LH_DEV f3 unit(f3 a) { return mul(a, rsqrt(dot3(a, a))); }
+
 // u = a / |a|: the gradient of a from the gradient of u.
 LH_DEV f3 normalize_bwd(f3 a, f3 gu) {
-    const float l = length3(a);
-    const f3 u = mul(a, 1.0f / l);
-    return mul(sub(gu, mul(u, dot3(u, gu))), 1.0f / l);
+    const float il = rsqrt(dot3(a, a));
+    const f3 u = mul(a, il);
+    return mul(sub(gu, mul(u, dot3(u, gu))), il);
 }
 
 struct BrdfGrad {
@@ -80,7 +88,7 @@ struct BrdfGrad {
 // upstream gradient g of its RGB output.
 LH_DEV f3 brdf_bwd(f3 v, f3 l, f3 n, float pr, float m, f3 base, f3 g, BrdfGrad& o) {
     const f3 hv = add(v, l);
-    const f3 h = normalize3(hv);
+    const f3 h = unit(hv);
     const f3 diffuse = mul(base, 1.0f - m);
     const float refl = (0.5f / 2.5f) * (0.5f / 2.5f);
     const float d0 = refl * (1.0f - m);
@@ -90,12 +98,15 @@ LH_DEV f3 brdf_bwd(f3 v, f3 l, f3 n, float pr, float m, f3 base, f3 g, BrdfGrad&
     const float NoH = clampf(nh, 0.0f, 1.0f), LoH = clampf(lh, 0.0f, 1.0f);
     const float a = pr * pr, a2 = a * a;
     const float fD = (NoH * a2 - NoH) * NoH + 1.0f;
-    const float D = a2 / ((kPi * fD) * fD);
+    const float iD = rcp((kPi * fD) * fD);
+    const float D = a2 * iD;
     const float p = pow5f(1.0f - LoH);
     const f3 F{f0.x + (1.0f - f0.x) * p, f0.y + (1.0f - f0.y) * p, f0.z + (1.0f - f0.z) * p};
-    const float sL = sqrtf((-NoL * a2 + NoL) * NoL + a2), sV = sqrtf((-NoV * a2 + NoV) * NoV + a2);
+    const float sL2 = (-NoL * a2 + NoL) * NoL + a2, sV2 = (-NoV * a2 + NoV) * NoV + a2;
+    const float isL = rsqrt(sL2), isV = rsqrt(sV2), sL = sL2 * isL, sV = sV2 * isV;
     const float den = NoL * sV + NoV * sL;
-    const float Vis = 0.5f / den;
+    const float iden = rcp(den);
+    const float Vis = 0.5f * iden;
     const float f90 = 0.5f + ((2.0f * a) * LoH) * LoH;
     const float q5L = pow5f(1.0f - NoL), q5V = pow5f(1.0f - NoV);
     const float FSL = 1.0f + (f90 - 1.0f) * q5L, FSV = 1.0f + (f90 - 1.0f) * q5V;
@@ -118,18 +129,18 @@ LH_DEV f3 brdf_bwd(f3 v, f3 l, f3 n, float pr, float m, f3 base, f3 g, BrdfGrad&
     float gNoV = gFSV * (f90 - 1.0f) * (-5.0f * pow4f(1.0f - NoV));
     float ga = gf90 * (2.0f * LoH * LoH);
     float gLoH = gf90 * (4.0f * a * LoH);
-    const float gden = -gVis * Vis / den;
+    const float gden = -gVis * Vis * iden;
     gNoV += gden * sL;
     gNoL += gden * sV;
     const float gsL = gden * NoV, gsV = gden * NoL;
-    gNoL += gsL * NoL * (1.0f - a2) / sL;
-    gNoV += gsV * NoV * (1.0f - a2) / sV;
-    float ga2 = gsL * (1.0f - NoL * NoL) / (2.0f * sL) + gsV * (1.0f - NoV * NoV) / (2.0f * sV);
+    gNoL += gsL * NoL * (1.0f - a2) * isL;
+    gNoV += gsV * NoV * (1.0f - a2) * isV;
+    float ga2 = gsL * (1.0f - NoL * NoL) * (0.5f * isL) + gsV * (1.0f - NoV * NoV) * (0.5f * isV);
     const f3 gf0 = mul(gF, 1.0f - p);
     const float gp = (gF.x * (1.0f - f0.x) + gF.y * (1.0f - f0.y)) + gF.z * (1.0f - f0.z);
     gLoH += gp * (-5.0f * pow4f(1.0f - LoH));
-    const float E = D / fD;
-    ga2 += gD * (1.0f / ((kPi * fD) * fD) - 2.0f * E * NoH * NoH);
+    const float E = D * rcp(fD);
+    ga2 += gD * (iD - 2.0f * E * NoH * NoH);
     const float gNoH = gD * (-2.0f * E * (2.0f * NoH * (a2 - 1.0f)));
     ga += ga2 * 2.0f * a;
     o.pr = ga * 2.0f * pr;
@@ -152,13 +163,6 @@ LH_DEV f3 brdf_bwd(f3 v, f3 l, f3 n, float pr, float m, f3 base, f3 g, BrdfGrad&
 struct TexSample {
     float val[4], du[4], dv[4];
 };
-LH_DEV f4 texel(const LhTexture& t, uint32_t level, int32_t w, int32_t h, int32_t i, int32_t j) {
-    i %= w;
-    if (i < 0) i += w;
-    j %= h;
-    if (j < 0) j += h;
-    return unpack_texel(t, texel_rgba8(t, level, uint32_t(w), uint32_t(i), uint32_t(j)));
-}
 // Adds weight times the bilinear sample at one level (and its uv derivatives).
 // The texel cell and its fractions come from FP64 coordinates: FP32 would
 // pick a neighbouring cell for uvs within rounding of a texel boundary.
@@ -168,10 +172,9 @@ LH_DEV void bilinear_acc(const LhTexture& t, uint32_t level, d2 uv, float weight
     const double fu = floor(u), fv = floor(vv);
     const float a = float(u - fu), b = float(vv - fv);
     const int32_t i = int32_t(fu), j = int32_t(fv);
-    const f4 t00 = texel(t, level, w, h, i, j), t10 = texel(t, level, w, h, i + 1, j);
-    const f4 t01 = texel(t, level, w, h, i, j + 1), t11 = texel(t, level, w, h, i + 1, j + 1);
-    const float c00[4] = {t00.x, t00.y, t00.z, t00.w}, c10[4] = {t10.x, t10.y, t10.z, t10.w};
-    const float c01[4] = {t01.x, t01.y, t01.z, t01.w}, c11[4] = {t11.x, t11.y, t11.z, t11.w};
+    const TexelQuad q = texel_quad(t, level, w, h, i, j);
+    const float c00[4] = {q.t00.x, q.t00.y, q.t00.z, q.t00.w}, c10[4] = {q.t10.x, q.t10.y, q.t10.z, q.t10.w};
+    const float c01[4] = {q.t01.x, q.t01.y, q.t01.z, q.t01.w}, c11[4] = {q.t11.x, q.t11.y, q.t11.z, q.t11.w};
     for (int k = 0; k < 4; k++) {
         const float top = c00[k] * (1.0f - a) + c10[k] * a, bot = c01[k] * (1.0f - a) + c11[k] * a;
         o.val[k] += weight * (top * (1.0f - b) + bot * b);
@@ -194,7 +197,7 @@ LH_DEV TexSample sample_d(const lh_scene& s, uint32_t index, const Lod& lod, d2 
 }
 
 struct LightGrad {
-    d3 sun_dir, sun_intensity;
+    f3 sun_dir, sun_intensity;  // a lane's sums over its pixels (a few per block)
 };
 
 // Vertex gradients of one tile are merged in an LDS hash table before they
@@ -203,13 +206,20 @@ struct LightGrad {
 // its position and normal gradients (six floats), keyed by the arena words of
 // both vec3s (position word | normal word << 32; ~0u for a vec3 outside the
 // arena), so a mesh info whose normals differ from another's over the same
-// positions gets entries of its own.  A tile claims at most 64 pixels x
-// 3 vertices = 192 entries: the 512-slot table stays below 38% load (short
-// probe chains) in 16 KB, so two one-wave blocks per SIMD fit in LDS.  Claimed
-// slots are listed (one LDS atomic per wave and probe round) so the flush
-// visits only those.
+// positions gets entries of its own.  The table is kept across the block's
+// tiles and flushed only when it holds more than kFlushAt entries after a tile,
+// and after the block's last tile: a tile (about eight triangles of about
+// eight pixels each) touches a dozen vertices, so most blocks flush once, and
+// the atomics of a flush, which a later load of the wave would wait behind,
+// leave the per-tile path.  A tile claims at most 64 pixels x 3 vertices =
+// 192 entries, so at most 352 of the 512 slots are taken (69% load) in 16 KB,
+// and two one-wave blocks per SIMD fit in LDS.  Claimed slots are listed (one
+// LDS atomic per wave and probe round) so the flush visits only those.
 constexpr uint32_t kHashSlots = 512;
-constexpr uint32_t kHashKeysMax = 192;
+constexpr uint32_t kTileClaims = 192;
+constexpr uint32_t kFlushAt = 160;
+constexpr uint32_t kHashKeysMax = kFlushAt + kTileClaims;
+static_assert(kHashKeysMax < kHashSlots, "an empty slot ends every probe");
 constexpr uint64_t kHashEmpty = ~0ull;
 constexpr uint32_t kNoWord = ~0u;
 struct VertexHash {
@@ -266,11 +276,13 @@ LH_DEV void add_vertex(const VertexHash& h, uint32_t slot, uint32_t part, f3 g) 
     atomicAdd(val + 2, g.z);
 }
 
-// Called by the whole wave after a tile: adds every merged vec3 to the arena
-// and empties the table.
-LH_DEV void flush_geometry(const lh_resolve_grads& gr, const VertexHash& h) {
+// Called by the whole wave after a tile: once the table holds more than
+// kFlushAt entries, or when `last`, adds every merged vec3 to the arena and
+// empties the table.
+LH_DEV void flush_geometry(const lh_resolve_grads& gr, const VertexHash& h, bool last) {
     __syncthreads();
     const uint32_t n = *h.count;
+    if (n <= kFlushAt && !last) return;
     for (uint32_t i = threadIdx.x; i < n; i += 64) {
         const uint32_t slot = h.used[i];
         const uint64_t key = h.keys[slot];
@@ -377,7 +389,7 @@ LH_DEV void pixel_bwd(const lh_scene& s, const lh_view& v, const lh_resolve_para
     const LhMat4& PV = u.combined_perspective_view;
     const d3 q{double(x) / double(u.window_size.x) * 2.0 - 1.0, double(y) / double(u.window_size.y) * 2.0 - 1.0, 1.0};
     d3 c[3];
-    double sumD, lam[3];
+    double sumD, invD, lam[3];
     f3 dpos[3], to_cam;  // world positions relative to vertex 0; camera - world
     {
         const LhMat4& T = in.transform;
@@ -399,7 +411,8 @@ LH_DEV void pixel_bwd(const lh_scene& s, const lh_view& v, const lh_resolve_para
         double Dk[3];
         for (int k = 0; k < 3; k++) Dk[k] = dot(q, cross(c[(k + 1) % 3], c[(k + 2) % 3]));
         sumD = (Dk[0] + Dk[1]) + Dk[2];
-        for (int k = 0; k < 3; k++) lam[k] = Dk[k] / sumD;
+        invD = 1.0 / sumD;
+        for (int k = 0; k < 3; k++) lam[k] = Dk[k] * invD;
         const d3 tc = D3(u.camera_pos) - ((pos[0] * lam[0] + pos[1] * lam[1]) + pos[2] * lam[2]);
         to_cam = {float(tc.x), float(tc.y), float(tc.z)};
         for (int j = 0; j < 3; j++) {
@@ -424,13 +437,13 @@ LH_DEV void pixel_bwd(const lh_scene& s, const lh_view& v, const lh_resolve_para
     f3 nraw[3], nn[3];
     for (int j = 0; j < 3; j++) {
         nraw[j] = load_normal(mi, sf.vi[j]);
-        nn[j] = normalize3(nraw[j]);
+        nn[j] = unit(nraw[j]);
     }
-    const f3 V = normalize3(to_cam);
+    const f3 V = unit(to_cam);
     const f3 nI = add(add(mul(nn[0], lamf[0]), mul(nn[1], lamf[1])), mul(nn[2], lamf[2]));
     const LhMat3& N = in.normal_transform;
     const f3 m3 = add(add(mul(mk3(N.c[0]), nI.x), mul(mk3(N.c[1]), nI.y)), mul(mk3(N.c[2]), nI.z));
-    const f3 nrm = mul(normalize3(m3), fz.flip);
+    const f3 nrm = mul(unit(m3), fz.flip);
     const f3 bcf = param_base_color(prm, mi, sf.mesh_info);
     const float rf = param_roughness(prm, s, sf.mesh_info), mf = param_metallic(prm, s, sf.mesh_info);
     const f3 sun_dir = param_sun_dir(prm, u), sun_int = param_sun_intensity(prm, u);
@@ -441,13 +454,14 @@ LH_DEV void pixel_bwd(const lh_scene& s, const lh_view& v, const lh_resolve_para
     const float roughness = tm.val[1] * rf, metallic = tm.val[2] * mf;
     const float shv = nrm.y * 0.025f + 0.025f;  // SH ambient: constant and y bands
     const float K = fz.K;
+    asm volatile("" ::: "memory");
     // ---- reverse ----
     // out = B * K * I + base * (1 - metallic) * shv;  B needs the upstream gB = g * I * K
     BrdfGrad bg;
     const f3 B = brdf_bwd(V, sun_dir, nrm, roughness, metallic, base, mul(mul(g, sun_int), K), bg);
     const f3 gI = mul(mul(g, B), K);
-    lg.sun_intensity = lg.sun_intensity + d3{gI.x, gI.y, gI.z};
-    lg.sun_dir = lg.sun_dir + d3{bg.l.x, bg.l.y, bg.l.z};
+    lg.sun_intensity = add(lg.sun_intensity, gI);
+    lg.sun_dir = add(lg.sun_dir, bg.l);
     const f3 gbase = add(mul(g, (1.0f - metallic) * shv), bg.base);
     const float gmet = -dot3(g, base) * shv + bg.metallic;
     f3 gnrm = bg.n;
@@ -479,7 +493,7 @@ LH_DEV void pixel_bwd(const lh_scene& s, const lh_view& v, const lh_resolve_para
     const double lg_dot = (glam[0] * lam[0] + glam[1] * lam[1]) + glam[2] * lam[2];
     d3 gc[3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
     for (int k = 0; k < 3; k++) {
-        const double gD = (glam[k] - lg_dot) / sumD;
+        const double gD = (glam[k] - lg_dot) * invD;
         const int a = (k + 1) % 3, b = (k + 2) % 3;
         gc[a] = gc[a] + cross(c[b], q) * gD;
         gc[b] = gc[b] + cross(q, c[a]) * gD;
@@ -512,7 +526,7 @@ LH_DEV void pixel_bwd(const lh_scene& s, const lh_view& v, const lh_resolve_para
 // Occupancy target (waves per SIMD) of the adjoint; 0 leaves the register
 // budget to the compiler.
 #ifndef LH_BACKWARD_WAVES
-#define LH_BACKWARD_WAVES 0
+#define LH_BACKWARD_WAVES 2
 #endif
 #if LH_BACKWARD_WAVES > 0
 #define LH_BACKWARD_OCCUPANCY LH_WAVES_ATTR(LH_BACKWARD_WAVES)
@@ -537,7 +551,7 @@ __global__ void __launch_bounds__(64) LH_BACKWARD_OCCUPANCY k_resolve_backward(l
     __syncthreads();
     const lh_view& v = vs.v;
     const uint32_t tiles_x = (v.width + 7) / 8, tiles = tiles_x * ((v.height + 7) / 8);
-    LightGrad lg{{0, 0, 0}, {0, 0, 0}};
+    LightGrad lg{{0.0f, 0.0f, 0.0f}, {0.0f, 0.0f, 0.0f}};
     // Material-factor gradients: lane i keeps the sums of the i-th distinct
     // mesh info the block met and adds them to the outputs once, at the end.
     // The few mesh infos that cover most of the view would otherwise take
@@ -560,19 +574,15 @@ __global__ void __launch_bounds__(64) LH_BACKWARD_OCCUPANCY k_resolve_backward(l
         float g_rf = 0.0f, g_mf = 0.0f;
         if (x < v.width && y < v.height)
             pixel_bwd<kRecorded>(s, v, prm, grad_radiance, gr, hash, x, y, mesh_info, g_bcf, g_rf, g_mf, lg);
-        if (gr.geometry) flush_geometry(gr, hash);
+        if (gr.geometry) flush_geometry(gr, hash, t + gridDim.x >= tiles);
         // material factors: one add per distinct mesh info of the wave
         uint64_t pending = ballot(mesh_info != ~0u);
         while (pending) {
             const uint32_t lead = __builtin_amdgcn_readlane(mesh_info, int(first_lane(pending)));
             const bool mine = mesh_info == lead;
-            double r[5] = {mine ? g_bcf.x : 0.0, mine ? g_bcf.y : 0.0, mine ? g_bcf.z : 0.0, mine ? g_rf : 0.0,
-                           mine ? g_mf : 0.0};
-            for (int i = 0; i < 5; i++) {
-                double sum = r[i];
-                for (int d = 32; d >= 1; d >>= 1) sum += __shfl_xor(sum, d, 64);
-                r[i] = sum;
-            }
+            const float r[5] = {wave_sum_f32(mine ? g_bcf.x : 0.0f), wave_sum_f32(mine ? g_bcf.y : 0.0f),
+                                wave_sum_f32(mine ? g_bcf.z : 0.0f), wave_sum_f32(mine ? g_rf : 0.0f),
+                                wave_sum_f32(mine ? g_mf : 0.0f)};
             // every lane holds the wave sums; pick the lane that keeps lead
             const uint64_t hit = ballot(mat_key == lead);
             uint32_t owner = 64;  // none: the table is full, add directly
@@ -583,10 +593,9 @@ __global__ void __launch_bounds__(64) LH_BACKWARD_OCCUPANCY k_resolve_backward(l
                 if (lane_id() == owner) mat_key = lead;
             }
             if (lane_id() == owner) {
-                for (int i = 0; i < 5; i++) mat_val[i] += float(r[i]);
+                for (int i = 0; i < 5; i++) mat_val[i] += r[i];
             } else if (owner == 64 && lane_id() == first_lane(pending)) {
-                const float val[5] = {float(r[0]), float(r[1]), float(r[2]), float(r[3]), float(r[4])};
-                add_material(lead, val);
+                add_material(lead, r);
             }
             pending &= ~ballot(mine);
         }
@@ -594,7 +603,7 @@ __global__ void __launch_bounds__(64) LH_BACKWARD_OCCUPANCY k_resolve_backward(l
     if (mat_key != ~0u) add_material(mat_key, mat_val);
     double r[6] = {lg.sun_dir.x, lg.sun_dir.y, lg.sun_dir.z, lg.sun_intensity.x, lg.sun_intensity.y,
                    lg.sun_intensity.z};
-    for (int i = 0; i < 6; i++) {
+    for (int i = 0; i < 6; i++) {  // the lanes' sums in FP64, in a fixed order
         double sum = r[i];
         for (int d = 32; d >= 1; d >>= 1) sum += __shfl_xor(sum, d, 64);
         r[i] = sum;

@@ -147,6 +147,14 @@ LH_DEV uint32_t wave_scan_add(uint32_t v) {
 LH_DEV int32_t wave_scan_max(int32_t v) {
     return wave_scan_dpp(v, INT32_MIN, [](int32_t a, int32_t b) { return max(a, b); });
 }
+// Sum of a float over the wave, returned to every lane: the DPP inclusive
+// scan leaves it in lane 63.  The wave must be converged.
+LH_DEV float wave_sum_f32(float v) {
+    const int32_t s = wave_scan_dpp(__float_as_int(v), 0, [](int32_t a, int32_t b) {
+        return __float_as_int(__int_as_float(a) + __int_as_float(b));
+    });
+    return __int_as_float(__builtin_amdgcn_readlane(s, 63));
+}
 LH_DEV uint32_t wave_sum_u32(uint32_t v) {
     for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
     return v;
