@@ -145,6 +145,64 @@ def raster_algorithmic_bytes(scene, draws, covered_pixels, sizes):
     return int(per_meshlet.sum() + 216 * len(np.unique(inst)) + 8 * covered_pixels)
 
 
+def tile_algorithmic_bytes(scene, view, sizes, covered_pixels):
+    """Compulsory HBM bytes of one launch of the tile raster (k_tile_raster):
+    per (meshlet, tile) entry the meshlet's snapped vertices from the bin cache
+    (12 per vertex), its triangles of the entry (4 each), the header words and
+    the entry itself (12); one 8-byte key per covered pixel written."""
+    cfg = view.cfg
+    L = abi.scratch_layout(cfg.width, cfg.height, view.dcap, cfg.big_capacity, cfg.clip_capacity, view.bcap)
+    t = view.handle[0]
+    base = view.offsets["scratch"]
+
+    def words(off, count):
+        return t[base + off: base + off + 4 * count].cpu().numpy().view(np.uint32)
+
+    n_items = int(view.counters()[abi.CNT_TILE_ITEMS])
+    if n_items == 0:
+        return 8 * covered_pixels
+    items = words(L["items"], 3 * n_items).reshape(-1, 3)
+    n_entries = int((items[:, 2] & 0x7FFFFFFF).max())
+    ent = words(L["entries"], n_entries)
+    g = (ent & ((1 << 25) - 1)).astype(np.int64)
+    part = (ent >> 25).astype(np.int64)
+    hdr = words(L["hdr"], 4 * (int(g.max()) + 1)).reshape(-1, 4)
+    field = hdr[g, 1].astype(np.int64)
+    ntri, lp = field & 255, (field >> 8) & 7
+    count = (((part + 1) * ntri) >> lp) - ((part * ntri) >> lp)
+    offs, nv_all, _ = sizes
+    draws = view.draws(0)[g]
+    mi = scene.instances["mesh_info_address"].astype(np.int64)[draws["instance_index"].astype(np.int64)]
+    nv = nv_all[offs[mi] + draws["meshlet_index"].astype(np.int64)]
+    return int((12 * nv + 4 * count + 12).sum() + 8 * covered_pixels)
+
+
+def resolve_algorithmic_bytes(scene, view, covered_pixels):
+    """Compulsory HBM bytes of one resolve launch, a lower bound: per pixel its
+    key (8), FP32 radiance (16) and BGRA8 (4); per distinct vertex the covered
+    pixels' triangles use, its position, normal, uv (32) and remap index (4).
+    Texel and shadow-map reads are not counted."""
+    from lighthugger_amd import abi
+
+    keys = view.visbuffer().ravel()
+    keys = keys[keys != np.uint64(abi.LH_VISBUFFER_CLEAR)]
+    mid = ((keys >> np.uint64(abi.LH_KEY_TRI_BITS)) & np.uint64((1 << 25) - 1)).astype(np.int64)
+    tri = (keys & np.uint64((1 << abi.LH_KEY_TRI_BITS) - 1)).astype(np.int64)
+    ref = scene.meshlet_table[mid]
+    mi = scene.instances["mesh_info_address"].astype(np.int64)[ref["instance_index"].astype(np.int64)]
+    n_vertices = 0
+    for m in np.unique(mi):
+        sel = mi == m
+        ml = scene.meshlets(int(m))[ref["meshlet_index"][sel].astype(np.int64)]
+        nt = int((ml["triangle_offset"].astype(np.int64) + 3 * ml["triangle_count"].astype(np.int64)).max())
+        micro = scene.micro(int(m), nt).astype(np.int64)
+        corners = micro[ml["triangle_offset"].astype(np.int64)[:, None] + 3 * tri[sel][:, None] + np.arange(3)]
+        idx = ml["index_offset"].astype(np.int64)[:, None] + corners
+        n_vertices += len(np.unique(idx))
+    pixels = view.cfg.width * view.cfg.height
+    return int(pixels * 28 + 36 * n_vertices)
+
+
 def batch_bench(scene, steps: int, n_cameras: int = 64):
     """Batched views: n_cameras seeded square cameras (side log-uniform in
     [256, 2048], multiples of 8), each with its own 4 cascades of the same side
@@ -411,13 +469,24 @@ def main():
         print("# profile-only run done", file=sys.stderr)
         return
 
-    # ---- timed region: exactly `steps` frames ----
+    # ---- timed region: exactly `steps` frames (lh_frame) ----
     torch.cuda.synchronize()
     t_start = time.perf_counter()
     for _ in range(args.steps):
         r.frame()
     torch.cuda.synchronize()
     ms = (time.perf_counter() - t_start) * 1000.0 / args.steps
+
+    # the same frames replayed from a hipGraph (lh_frame_graph), captured
+    # before its timed loop
+    r.frame_graph()
+    torch.cuda.synchronize()
+    t_graph = time.perf_counter()
+    for _ in range(args.steps):
+        r.frame_graph()
+    torch.cuda.synchronize()
+    ms_graph = (time.perf_counter() - t_graph) * 1000.0 / args.steps
+    r.close_graph()
 
     # the same frame with the occlusion culling switched the other way (same output)
     r.view.c.flags ^= abi.VIEW_HIZ
@@ -462,7 +531,8 @@ def main():
     atomics = [int(x) for x in sc[abi.CNT_ATOMICS: abi.CNT_ATOMICS + abi.NUM_PASSES]]
     raster_atomics = {"fragments_per_pass": frags, "atomics_per_pass": atomics}
     for kind, lo, hi in (("view", 0, 1), ("shadow", 1, abi.NUM_PASSES)):
-        kt = sum(per_frame.get(f"raster_{k}_{kind}", 0.0) for k in ("meshlets", "clip", "big"))
+        kt = sum(per_frame.get(f"{k}_{kind}", 0.0)
+                 for k in ("raster_meshlets", "raster_clip", "raster_big", "tile_raster", "raster_direct"))
         if kt > 0:
             raster_atomics[f"{kind}_atomics_per_s_G"] = round(sum(atomics[lo:hi]) / (kt * 1e-3) / 1e9, 2)
     if not args.no_atomic_probe:
@@ -471,7 +541,28 @@ def main():
         raster_atomics["ceiling_u32_shadow_G"] = atomic_ceiling(shadow, shadow, 32)
 
     roofline = None
-    if dominant.startswith("raster_meshlets"):
+    formulas = ("raster_meshlets_view", "raster_meshlets_shadow", "tile_raster_view", "resolve")
+    known = [k for k in formulas if k in per_frame]
+    if dominant not in known and known:
+        dominant = max(known, key=per_frame.get)  # the costliest kernel whose work is counted
+    if dominant in ("tile_raster_view", "resolve"):
+        if dominant == "tile_raster_view":
+            algo = tile_algorithmic_bytes(scene, r.view, sizes, covered)
+        else:
+            algo = resolve_algorithmic_bytes(scene, r.view, covered)
+        launches = max(1, round(len(acc[dominant]) / args.steps))
+        algo //= launches
+        achieved = algo / (per_launch[dominant] * 1e-3) / 1e9
+        traffic = None
+        tfile = ROOT / "profiles" / "traffic.json"
+        if tfile.exists():
+            entry = json.loads(tfile.read_text()).get(dominant)
+            traffic = entry["bytes"] if entry else None
+        roofline = {"kernel": dominant, "bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
+                    "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
+                    "algorithmic_bytes": algo, "avg_launch_ms": round(per_launch[dominant], 5),
+                    "launches_per_frame": launches}
+    elif dominant.startswith("raster_meshlets"):
         if dominant.endswith("_view"):
             algo = raster_algorithmic_bytes(scene, r.view.draws(0), covered, sizes)
         else:
@@ -537,6 +628,7 @@ def main():
         "steps": args.steps,
         "warmup": args.warmup,
         "ms_per_step": round(ms, 4),
+        "ms_per_step_graph_replay": round(ms_graph, 4),
         "higher_is_better": False,
         "scaling": "weak",
         "vs_baseline": None,

@@ -263,6 +263,20 @@ lh_status lh_frame(const lh_scene* scene, const lh_view* view, lh_stream stream)
 #define LH_STATIC_SCRATCH_BYTES (64ull << 20)
 lh_status lh_frame_noalloc(const lh_scene* scene, const lh_view* view, lh_stream stream);
 
+/* lh_frame captured once into a hipGraph and replayed: one graph launch per
+ * frame instead of the frame's separate kernel launches, with bit-identical
+ * results.  It plays the part of the reference's per-frame command buffer
+ * (src/rendering.cpp:28-548, re-recorded there every frame).  The scene and
+ * view structs are captured by value at lh_frame_graph_create: the device
+ * memory they point to (uniforms and camera, instances, buffers) may change
+ * between runs, but changing a field of either struct needs a new graph.
+ * Capture happens on a stream the graph owns (default priority); runs may
+ * use any stream of the same device. */
+typedef struct lh_frame_graph lh_frame_graph;
+lh_status lh_frame_graph_create(const lh_scene* scene, const lh_view* view, lh_frame_graph** out);
+lh_status lh_frame_graph_run(lh_frame_graph* graph, lh_stream stream);
+void lh_frame_graph_destroy(lh_frame_graph* graph);
+
 /* lh_frame with per-kernel device times: an event is recorded on `stream`
  * before every kernel launch, the call synchronizes on the last one and
  * writes one entry per kernel (in launch order).  No counterpart in the
@@ -313,8 +327,13 @@ typedef struct lh_tuning {
                                   the triangle in place; larger ones go to the big-triangle queue */
     uint32_t tile_item_entries; /* tiled main-view raster: binned meshlets per work item (8..512; 0 = 64) */
     uint32_t tile_blocks;       /* tiled main-view raster: persistent grid of 512-thread blocks (0 = 1024) */
+    uint32_t tile_raster;       /* 1: the main view goes through the tiled raster (when its bin_capacity
+                                   is non-zero); 0: the per-meshlet raster.  The tiled raster wins when
+                                   meshlets cover many pixels each; with a few pixels per meshlet (small
+                                   views of dense scenes) the per-meshlet raster is cheaper. */
 } lh_tuning;
-/* Sets / reads every class at once (lh_set_tuning) or one class.  Classes by
+/* lh_set_tuning sets every class at once, lh_get_tuning reads class 1;
+ * lh_set_tuning_class / lh_get_tuning_class set or read one class.  Classes by
  * main-view pixels: 0 < 0.6 MP, 1 < 2.5 MP, 2 < 9 MP, 3 larger; class 4 is used
  * by batched launches (lh_batch_run with LH_BATCH_BATCHED). */
 #define LH_TUNING_CLASSES 5

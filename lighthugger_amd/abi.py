@@ -168,19 +168,35 @@ LH_OK, LH_ERR_INVALID, LH_ERR_CAPACITY, LH_ERR_HIP, LH_ERR_UNSUPPORTED = 0, 1, 2
 
 TILE_SIDE, TILE_MAX_RECT, BIN_CACHE_BYTES = 64, 16, 1280
 STATIC_SCRATCH_BYTES = 64 << 20  # LH_STATIC_SCRATCH_BYTES: the scratch array of lh_frame_noalloc
+TUNING_CLASSES = 5  # LH_TUNING_CLASSES
 CNT_TILE_ITEMS, CNT_TILE_DIRECT = 56, 58
+
+
+def scratch_layout(width: int, height: int, draw_capacity: int, big_capacity: int, clip_capacity: int,
+                   bin_capacity: int) -> dict:
+    """Byte offsets of the parts of lh_view.scratch (include/lh/lh.h): raster
+    queues, then the tiled raster's bins when bin_capacity > 0, each part
+    256-byte aligned; "end" is lh_scratch_bytes."""
+    a = lambda n: (n + 255) & ~255  # noqa: E731
+    parts = [("clip", clip_capacity * CLIP_ENTRY_BYTES), ("big", big_capacity * BIG_ENTRY_BYTES)]
+    out = {}
+    if bin_capacity:
+        tiles_x, tiles_y = (width + TILE_SIDE - 1) // TILE_SIDE, (height + TILE_SIDE - 1) // TILE_SIDE
+        entries = TILE_MAX_RECT * bin_capacity
+        items = tiles_x * tiles_y + entries // 8
+        parts += [("hdr", bin_capacity * 16), ("cache", bin_capacity * BIN_CACHE_BYTES),
+                  ("tile_count", tiles_x * tiles_y * 4), ("tile_cursor", tiles_x * tiles_y * 4),
+                  ("items", items * 12), ("entries", entries * 4), ("direct", draw_capacity * 4)]
+        out.update(tiles_x=tiles_x, tiles_y=tiles_y, entry_capacity=entries, item_capacity=items)
+    o = 0
+    for name, nb in parts:
+        out[name] = o
+        o += a(nb)
+    out["end"] = o
+    return out
 
 
 def scratch_bytes(width: int, height: int, draw_capacity: int, big_capacity: int, clip_capacity: int,
                   bin_capacity: int) -> int:
-    """lh_scratch_bytes (include/lh/lh.h): raster queues, then the tiled
-    raster's bins when bin_capacity > 0, each part 256-byte aligned."""
-    a = lambda n: (n + 255) & ~255  # noqa: E731
-    total = a(clip_capacity * CLIP_ENTRY_BYTES) + a(big_capacity * BIG_ENTRY_BYTES)
-    if bin_capacity:
-        tiles = ((width + TILE_SIDE - 1) // TILE_SIDE) * ((height + TILE_SIDE - 1) // TILE_SIDE)
-        entries = TILE_MAX_RECT * bin_capacity
-        items = tiles + entries // 8
-        total += (a(bin_capacity * 16) + a(bin_capacity * BIN_CACHE_BYTES) + 2 * a(tiles * 4) + a(items * 12)
-                  + a(entries * 4) + a(draw_capacity * 4))
-    return total
+    """lh_scratch_bytes (include/lh/lh.h)."""
+    return scratch_layout(width, height, draw_capacity, big_capacity, clip_capacity, bin_capacity)["end"]

@@ -30,10 +30,11 @@ SPACE = {
     "cull_blocks": [512, 1024, 2048],
     "tile_item_entries": [32, 64, 128],
     "tile_blocks": [768, 1024, 2048],
+    "tile_raster": [0, 1],
 }
 FIELDS = ("small_extent_px", "meshlet_blocks", "big_blocks", "clip_blocks", "cull_blocks", "medium_extent_px",
-          "tile_item_entries", "tile_blocks")
-DEFAULTS = {"tile_item_entries": 64, "tile_blocks": 1024}  # fields absent from older tuning files
+          "tile_item_entries", "tile_blocks", "tile_raster")
+DEFAULTS = {"tile_item_entries": 64, "tile_blocks": 1024, "tile_raster": 1}  # fields absent from older files
 
 
 def _tuning(d: dict):

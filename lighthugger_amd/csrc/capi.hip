@@ -167,13 +167,14 @@ lh_status lh_set_tuning_class(uint32_t cls, const lh_tuning* t) {
     if (cls >= LH_TUNING_CLASSES || !t || t->small_extent_px == 0 || t->small_extent_px > 32 ||
         t->meshlet_blocks == 0 || t->big_blocks == 0 || t->clip_blocks == 0 || t->cull_blocks == 0 ||
         t->medium_extent_px < t->small_extent_px || t->medium_extent_px > 64 ||
-        (t->tile_item_entries && (t->tile_item_entries < 8 || t->tile_item_entries > 512)))
+        (t->tile_item_entries && (t->tile_item_entries < 8 || t->tile_item_entries > 512)) || t->tile_raster > 1)
         return LH_ERR_INVALID;
     lhk::Tuning& g = lhk::g_tunings[cls];
     g.raster.small_extent_px = t->small_extent_px;
     g.raster.medium_extent_px = t->medium_extent_px;
     g.raster.tile_item_entries = t->tile_item_entries ? t->tile_item_entries : 64u;
     g.raster.tile_blocks = t->tile_blocks ? t->tile_blocks : 1024u;
+    g.raster.tile_raster = t->tile_raster;
     g.raster.meshlet_blocks = t->meshlet_blocks;
     g.raster.big_blocks = t->big_blocks;
     g.raster.clip_blocks = t->clip_blocks;
@@ -188,6 +189,7 @@ lh_status lh_get_tuning_class(uint32_t cls, lh_tuning* t) {
     t->medium_extent_px = g.raster.medium_extent_px;
     t->tile_item_entries = g.raster.tile_item_entries;
     t->tile_blocks = g.raster.tile_blocks;
+    t->tile_raster = g.raster.tile_raster;
     t->meshlet_blocks = g.raster.meshlet_blocks;
     t->big_blocks = g.raster.big_blocks;
     t->clip_blocks = g.raster.clip_blocks;
@@ -203,6 +205,8 @@ lh_status lh_set_tuning(const lh_tuning* t) {
     return LH_OK;
 }
 
+// Class 1 (views of 0.6 to 2.5 MP, 1080p among them); lh_get_tuning_class
+// reads any one class.
 lh_status lh_get_tuning(lh_tuning* t) { return lh_get_tuning_class(1, t); }
 
 uint32_t lh_tuning_class_of(uint32_t width, uint32_t height) { return lhk::tuning_class(width, height); }

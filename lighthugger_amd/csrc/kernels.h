@@ -21,6 +21,7 @@ struct RasterTuning {
     uint32_t clip_blocks = 256;     // persistent grid of the clip kernel (64-thread blocks)
     uint32_t tile_item_entries = 64; // tiled main-view raster: binned meshlets per work item (>= 8)
     uint32_t tile_blocks = 1024;    // tiled main-view raster: persistent grid (512-thread blocks)
+    uint32_t tile_raster = 1;       // main view through the tiled raster (when bin_capacity > 0)
 };
 
 struct Tuning {

@@ -323,14 +323,15 @@ namespace lhk {
 
 // The per-meshlet raster of a pass set and its queue launches.  A single
 // view's main pass with bin_capacity > 0 goes through the tiled raster first
-// (tile.hip); the per-meshlet kernel then covers only the draws it left.
+// (tile.hip) when its tuning class asks for it; the per-meshlet kernel then
+// covers only the draws it left.
 template <int KIND, bool ST, class VS>
 hipError_t launch_kernels(const lh_scene& s, const VS& vs, PassSet ps, const RasterTuning& t, uint32_t clip_blocks,
                           uint32_t big_blocks, hipStream_t st) {
     const uint32_t nv = vs.count();
     bool tiled = false;
     if constexpr (std::is_same<VS, OneView>::value)
-        tiled = KIND == 0 && vs.v.bin_capacity > 0;
+        tiled = KIND == 0 && vs.v.bin_capacity > 0 && t.tile_raster != 0;
     if (tiled) {
         if constexpr (std::is_same<VS, OneView>::value) {
             hipError_t e = raster_tiles<ST>(s, vs, ps, t, st);

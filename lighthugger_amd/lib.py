@@ -31,7 +31,7 @@ class Tuning(C.Structure):
 
     _fields_ = [("small_extent_px", C.c_uint32), ("meshlet_blocks", C.c_uint32), ("big_blocks", C.c_uint32),
                 ("clip_blocks", C.c_uint32), ("cull_blocks", C.c_uint32), ("medium_extent_px", C.c_uint32),
-                ("tile_item_entries", C.c_uint32), ("tile_blocks", C.c_uint32)]
+                ("tile_item_entries", C.c_uint32), ("tile_blocks", C.c_uint32), ("tile_raster", C.c_uint32)]
 
     def as_dict(self) -> dict:
         return {f: int(getattr(self, f)) for f, _ in self._fields_}
@@ -69,6 +69,12 @@ def lib() -> C.CDLL:
         L.lh_version.restype = C.c_char_p
         L.lh_frame_noalloc.restype = C.c_int32
         L.lh_frame_noalloc.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
+        L.lh_frame_graph_create.restype = C.c_int32
+        L.lh_frame_graph_create.argtypes = [C.c_void_p, C.c_void_p, C.POINTER(C.c_void_p)]
+        L.lh_frame_graph_run.restype = C.c_int32
+        L.lh_frame_graph_run.argtypes = [C.c_void_p, C.c_void_p]
+        L.lh_frame_graph_destroy.restype = None
+        L.lh_frame_graph_destroy.argtypes = [C.c_void_p]
         L.lh_frame_profiled.restype = C.c_int32
         L.lh_frame_profiled.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(KernelTime), C.c_uint32,
                                         C.POINTER(C.c_uint32)]
@@ -116,6 +122,21 @@ def set_tuning(t: Tuning) -> None:
     check(lib().lh_set_tuning(C.byref(t)), "lh_set_tuning")
 
 
+def get_tuning_classes() -> list[Tuning]:
+    """Every class's tuning (lh_get_tuning_class), e.g. to restore them later."""
+    out = []
+    for c in range(abi.TUNING_CLASSES):
+        t = Tuning()
+        check(lib().lh_get_tuning_class(c, C.byref(t)), "lh_get_tuning_class")
+        out.append(t)
+    return out
+
+
+def set_tuning_classes(ts: list[Tuning]) -> None:
+    for c, t in enumerate(ts):
+        check(lib().lh_set_tuning_class(c, C.byref(t)), "lh_set_tuning_class")
+
+
 class Renderer:
     """A scene resident on the GPU plus one view's working set."""
 
@@ -124,6 +145,7 @@ class Renderer:
         self.scene = SceneData(scene, self.mem)
         self.view = ViewData(cfg, self.scene, self.mem)
         self.host_scene = scene
+        self._graph = None
 
     def set_camera(self, cam: abi.Camera) -> None:
         self.view.set_uniforms(make_uniforms(cam, self.host_scene.num_instances), self.scene)
@@ -134,6 +156,22 @@ class Renderer:
 
     def frame(self) -> None:
         self.call("lh_frame")
+
+    def frame_graph(self) -> None:
+        """lh_frame replayed from a hipGraph (lh_frame_graph), captured on the
+        first call.  The camera may change between calls (set_camera writes
+        the device uniforms the graph reads)."""
+        if self._graph is None:
+            h = C.c_void_p()
+            check(lib().lh_frame_graph_create(self.scene.address, self.view.address, C.byref(h)),
+                  "lh_frame_graph_create")
+            self._graph = h
+        check(lib().lh_frame_graph_run(self._graph, current_stream()), "lh_frame_graph_run")
+
+    def close_graph(self) -> None:
+        if self._graph is not None:
+            lib().lh_frame_graph_destroy(self._graph)
+            self._graph = None
 
     def frame_profiled(self) -> list[tuple[str, float]]:
         """One frame with per-kernel device times (ms), in launch order."""

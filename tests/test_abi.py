@@ -99,7 +99,7 @@ def test_library_exports_every_declared_function(lib_name, header):
 
 
 def test_host_only_entry_points_without_gpu():
-    from lighthugger_amd.lib import get_tuning, lib, set_tuning
+    from lighthugger_amd.lib import get_tuning, get_tuning_classes, lib, set_tuning_classes
 
     L = lib()
     assert L.lh_version().startswith(b"lighthugger_amd")
@@ -110,12 +110,16 @@ def test_host_only_entry_points_without_gpu():
     assert L.lh_scratch_bytes(C.byref(v)) == abi.scratch_bytes(1920, 1080, 5000, 10, 100, 3000)
     assert abi.scratch_bytes(1920, 1080, 5000, 10, 100, 3000) > 3000 * abi.BIN_CACHE_BYTES
     t = get_tuning()
-    assert 1 <= t.small_extent_px <= 32
+    assert 1 <= t.small_extent_px <= 32 and t.tile_raster in (0, 1)
     bad = Tuning(0, 1, 1, 1, 1, 8)
     assert L.lh_set_tuning(C.byref(bad)) == abi.LH_ERR_INVALID
     bad = Tuning(8, 1, 1, 1, 1, 4)  # medium below small
     assert L.lh_set_tuning(C.byref(bad)) == abi.LH_ERR_INVALID
-    set_tuning(t)
+    bad = Tuning(8, 1, 1, 1, 1, 8, 0, 0, 2)  # tile_raster is 0 or 1
+    assert L.lh_set_tuning(C.byref(bad)) == abi.LH_ERR_INVALID
+    classes = get_tuning_classes()
+    set_tuning_classes(classes)
+    assert [c.as_dict() for c in get_tuning_classes()] == [c.as_dict() for c in classes]
     # argument validation happens before any HIP call
     view = abi.LhView()
     scene = abi.LhScene()

@@ -263,7 +263,7 @@ def test_output_independent_of_tuning():
     radiance are bit-identical across extreme settings."""
     import torch
 
-    from lighthugger_amd.lib import Renderer, Tuning, get_tuning, set_tuning
+    from lighthugger_amd.lib import Renderer, Tuning, get_tuning_classes, set_tuning, set_tuning_classes
 
     scene = HostScene.preset(abi.PRESET_LARGE, texture_size=64)
     cam = preset_camera(abi.PRESET_LARGE)
@@ -271,11 +271,11 @@ def test_output_independent_of_tuning():
     from lighthugger_amd.scene import make_camera
 
     r.set_camera(make_camera(cam.position.tolist(), cam.target.tolist(), 960, 540))
-    saved = get_tuning()
+    saved = get_tuning_classes()
     ref = None
     try:
-        for t in (Tuning(1, 8192, 2048, 256, 1024, 1, 8, 256), Tuning(2, 4096, 512, 64, 512, 16, 512, 2048),
-                  Tuning(8, 16384, 2048, 256, 2048, 64, 32, 64), Tuning(32, 1024, 64, 16, 128, 64, 0, 0)):
+        for t in (Tuning(1, 8192, 2048, 256, 1024, 1, 8, 256, 1), Tuning(2, 4096, 512, 64, 512, 16, 512, 2048, 1),
+                  Tuning(8, 16384, 2048, 256, 2048, 64, 32, 64, 0), Tuning(32, 1024, 64, 16, 128, 64, 0, 0, 1)):
             set_tuning(t)
             r.frame()
             torch.cuda.synchronize()
@@ -286,11 +286,11 @@ def test_output_independent_of_tuning():
             for k in ("visbuffer", "shadow", "radiance", "ldr"):
                 assert np.array_equal(ref[k], s[k]), (t.as_dict(), k)
     finally:
-        set_tuning(saved)
+        set_tuning_classes(saved)
 
 
 @pytest.mark.parametrize("hiz", [False, True])
-def test_tiled_raster_equals_per_meshlet_raster(hiz):
+def test_tiled_raster_equals_per_meshlet_raster(hiz, tiled_raster):
     """The main view through the LDS-binned tile raster (every surviving
     meshlet binned), partly binned (a bin capacity of 2,000 draws: the rest
     take the per-meshlet raster) and not at all (bin_capacity 0): identical

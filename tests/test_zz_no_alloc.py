@@ -40,7 +40,7 @@ def _same(a, b, what):
             assert np.array_equal(a[k], b[k]), f"{what}: {k} differs"
 
 
-def test_noalloc_frame_equals_lh_frame_two_calls():
+def test_noalloc_frame_equals_lh_frame_two_calls(tiled_raster):
     from lighthugger_amd.lib import Renderer
 
     scene = HostScene.preset(abi.PRESET_SMALL, num_instances=48)
