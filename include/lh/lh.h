@@ -60,6 +60,8 @@ typedef void* lh_stream; /* hipStream_t */
 #define LH_CNT_TILE_ITEMS 56   /* tiled main-view raster: work items (tile, run of binned meshlets) of the last launch */
 #define LH_CNT_TILE_DIRECT 58  /* tiled main-view raster: meshlets left to the per-meshlet raster (beyond
                                   bin_capacity, or a screen rectangle over LH_TILE_MAX_RECT tiles) */
+#define LH_CNT_BACKWARD_TICKET 60 /* lh_resolve_backward: blocks done; zero between calls (the view reset
+                                     zeroes it, the call's last block sets it back to zero) */
 #define LH_NUM_COUNTERS 64
 #define LH_OVERFLOW_INSTANCES 0x1u
 #define LH_OVERFLOW_DRAWS 0x2u
@@ -229,7 +231,11 @@ lh_status lh_resolve_ex(const lh_scene* scene, const lh_view* view, const lh_res
  * Held fixed, like visibility: the back-face flip, the cascade choice and PCF
  * shadow factor, and the texture LOD (mip levels and their blend weight).
  * Mesh infos with 16-bit vertices, a normal texture or alpha clip are skipped
- * and flagged with LH_FLAG_BACKWARD_UNSUPPORTED in the view's counters. */
+ * and flagged with LH_FLAG_BACKWARD_UNSUPPORTED in the view's counters.  The
+ * view's LH_CNT_BACKWARD_TICKET counter must be zero when the call starts, as
+ * the view reset and every completed call leave it; calls on one view must not
+ * overlap.  The view's scratch holds the per-block partial sums (it must be
+ * at least 128 KiB). */
 typedef struct lh_resolve_grads {
     float* base_color_factor; /* [num_mesh_infos * 3] */
     float* roughness_factor;  /* [num_mesh_infos] */

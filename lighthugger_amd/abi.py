@@ -170,6 +170,7 @@ TILE_SIDE, TILE_MAX_RECT, BIN_CACHE_BYTES = 64, 16, 1280
 STATIC_SCRATCH_BYTES = 64 << 20  # LH_STATIC_SCRATCH_BYTES: the scratch array of lh_frame_noalloc
 TUNING_CLASSES = 5  # LH_TUNING_CLASSES
 CNT_TILE_ITEMS, CNT_TILE_DIRECT = 56, 58
+CNT_BACKWARD_TICKET = 60  # LH_CNT_BACKWARD_TICKET: zero between lh_resolve_backward calls
 
 
 def scratch_layout(width: int, height: int, draw_capacity: int, big_capacity: int, clip_capacity: int,

@@ -26,7 +26,7 @@
 // gradients are summed over the wave per distinct mesh info and kept per block
 // in registers (one lane per mesh info) until the block ends;
 // the six light gradients stay in registers and every block writes one
-// partial, which a second one-block kernel sums in a fixed order.
+// partial, which the last block to finish sums in a fixed order.
 #include "kernels.h"
 #include "shading.h"
 
@@ -614,29 +614,30 @@ __global__ void __launch_bounds__(64) LH_BACKWARD_OCCUPANCY k_resolve_backward(l
             if (threadIdx.x == uint32_t(i)) val = r[i];
         partials[size_t(blockIdx.x) * 8 + threadIdx.x] = val;
     }
-}
-
-// Sums the per-block light partials with a fixed reduction tree (the result
-// does not depend on timing) and adds them to the outputs: 256 threads each
-// sum a strided subset per component, then a tree in LDS.
-__global__ void __launch_bounds__(256) k_reduce_light(const double* partials, uint32_t blocks, lh_resolve_grads gr) {
-    __shared__ double acc[6][256];
-    const uint32_t t = threadIdx.x;
-    for (int i = 0; i < 6; i++) {
-        double sum = 0.0;
-        for (uint32_t b = t; b < blocks; b += 256) sum += partials[size_t(b) * 8 + i];
-        acc[i][t] = sum;
+    // The last block to finish sums every block's partial, lane l over blocks
+    // l, l + 64, ... then across the lanes in a fixed pattern, so the result
+    // does not depend on which block that is or on timing, and adds the sums
+    // to the outputs.
+    __threadfence();  // this block's partial before its ticket
+    uint32_t ticket = 0;
+    if (threadIdx.x == 0) ticket = atomicAdd(&v.counters[LH_CNT_BACKWARD_TICKET], 1u);
+    ticket = __shfl(ticket, 0, 64);
+    if (ticket != gridDim.x - 1) return;
+    __threadfence();  // every ticket before the partials are read
+    double acc[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    for (uint32_t b = threadIdx.x; b < gridDim.x; b += 64)
+        for (int i = 0; i < 6; i++)
+            acc[i] += __hip_atomic_load(partials + size_t(b) * 8 + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    for (int i = 0; i < 6; i++)
+        for (int d = 32; d >= 1; d >>= 1) acc[i] += __shfl_xor(acc[i], d, 64);
+    if (threadIdx.x < 6) {
+        float* dst = threadIdx.x < 3 ? gr.sun_dir : gr.sun_intensity;
+        double val = 0.0;
+        for (int i = 0; i < 6; i++)
+            if (threadIdx.x == uint32_t(i)) val = acc[i];
+        if (dst) dst[threadIdx.x % 3] += float(val);
     }
-    __syncthreads();
-    for (uint32_t w = 128; w >= 1; w >>= 1) {
-        if (t < w)
-            for (int i = 0; i < 6; i++) acc[i][t] += acc[i][t + w];
-        __syncthreads();
-    }
-    if (t < 6) {
-        float* dst = t < 3 ? gr.sun_dir : gr.sun_intensity;
-        if (dst) dst[t % 3] += float(acc[t][0]);
-    }
+    if (threadIdx.x == 0) v.counters[LH_CNT_BACKWARD_TICKET] = 0;  // ready for the next call
 }
 
 }  // namespace
@@ -658,8 +659,6 @@ hipError_t resolve_backward(const lh_scene& s, const OneView& vs, const lh_resol
     else
         hipLaunchKernelGGL(k_resolve_backward<false>, dim3(blocks), dim3(64), 0, st, s, vs, params, grad_radiance,
                            grads, partials);
-    mark("reduce_light", st);
-    hipLaunchKernelGGL(k_reduce_light, dim3(1), dim3(256), 0, st, partials, blocks, grads);
     return hipGetLastError();
 }
 
