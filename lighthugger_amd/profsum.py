@@ -24,6 +24,14 @@ from pathlib import Path
 
 # Kernel symbols that are unique to one entry of lh_frame_profiled's names.
 FRAME_KERNELS = {
+    "tile_raster_view": "k_tile_raster<false>",
+    "bin_meshlets_view": "k_bin_meshlets<0>",
+    "bin_count_view": "k_bin_count",
+    "bin_scan_view": "k_bin_scan",
+    "bin_fill_view": "k_bin_fill",
+    # with the tiled raster on, the per-meshlet kernel of the main view runs as
+    # raster_direct_view (the draws the bins left); otherwise as raster_meshlets_view
+    "raster_direct_view": "k_raster_meshlets<0, lhd::OneView, false, false>",
     "raster_meshlets_view": "k_raster_meshlets<0, lhd::OneView, false, false>",
     "raster_meshlets_shadow": "k_raster_meshlets<1, lhd::OneView, false, false>",
     "raster_big_view": "k_raster_big<0, lhd::OneView, false, false>",
