@@ -374,6 +374,17 @@ def batch_profile(scene, n_cameras: int = 64, repeats: int = 3) -> dict:
     return out
 
 
+def committed_traffic(kernel: str, workload: str):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 counter
+    runs (FETCH_SIZE, WRITE_SIZE scaled by the copy kernel's), which were
+    taken on the large workload; None for another workload or kernel."""
+    tfile = ROOT / "profiles" / "traffic.json"
+    if workload != "large" or not tfile.exists():
+        return None
+    entry = json.loads(tfile.read_text()).get(kernel)
+    return entry["bytes"] if entry else None
+
+
 def _tuning_of(cls: int) -> dict:
     import ctypes as C
 
@@ -553,11 +564,7 @@ def main():
         launches = max(1, round(len(acc[dominant]) / args.steps))
         algo //= launches
         achieved = algo / (per_launch[dominant] * 1e-3) / 1e9
-        traffic = None
-        tfile = ROOT / "profiles" / "traffic.json"
-        if tfile.exists():
-            entry = json.loads(tfile.read_text()).get(dominant)
-            traffic = entry["bytes"] if entry else None
+        traffic = committed_traffic(dominant, args.workload)
         roofline = {"kernel": dominant, "bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
                     "algorithmic_bytes": algo, "avg_launch_ms": round(per_launch[dominant], 5),
@@ -578,13 +585,7 @@ def main():
         if dominant.endswith("_view"):
             algo //= launches
         achieved = algo / (per_launch[dominant] * 1e-3) / 1e9
-        # HBM bytes per launch from the committed rocprofv3 counter runs
-        # (FETCH_SIZE, WRITE_SIZE scaled by the copy kernel's), when present
-        traffic = None
-        tfile = ROOT / "profiles" / "traffic.json"
-        if tfile.exists():
-            entry = json.loads(tfile.read_text()).get(dominant)
-            traffic = entry["bytes"] if entry else None
+        traffic = committed_traffic(dominant, args.workload)
         roofline = {"kernel": dominant, "bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
                     "algorithmic_bytes": algo, "avg_launch_ms": round(per_launch[dominant], 5),
