@@ -544,9 +544,41 @@ LH_DEV float shadow_factor(const lh_view& v, const MeshInfo& mi, f3 world, uint3
     const uint32_t layer = min(cascade, 3u);
     float shadow_sum = 0.0f;
     const float S = float(v.shadow_size);
+    const float ref = sc.z - bias;
+    // The nine taps' 2x2 cells normally tile the 4x4 texels around the centre
+    // tap's cell: those 16 comparisons are made once (a bit each), and a tap
+    // uses them when its own cell, computed exactly as shadow_tap computes
+    // it, is the expected one; a tap that rounding puts elsewhere reads its
+    // own texels.  The sum is the same, bit for bit.
+    const int32_t N = int32_t(v.shadow_size);
+    const uint32_t* map = v.shadow_maps + size_t(layer) * v.shadow_size * v.shadow_size;
+    const int32_t i0 = int32_t(floorf(sc.x * S - 0.5f)), j0 = int32_t(floorf(sc.y * S - 0.5f));
+    uint32_t lit = 0;  // bit 4 dy + dx: texel (i0 - 1 + dx, j0 - 1 + dy) passes the compare
+#pragma unroll
+    for (int dy = 0; dy < 4; dy++) {
+        const int32_t y = min(max(j0 - 1 + dy, 0), N - 1);
+#pragma unroll
+        for (int dx = 0; dx < 4; dx++) {
+            const int32_t x = min(max(i0 - 1 + dx, 0), N - 1);
+            lit |= (ref < __uint_as_float(map[size_t(y) * uint32_t(N) + uint32_t(x)]) ? 1u : 0u) << (4 * dy + dx);
+        }
+    }
+#pragma unroll
     for (int ox = -1; ox <= 1; ox++)
-        for (int oy = -1; oy <= 1; oy++)
-            shadow_sum += shadow_tap(v, layer, f2{sc.x + float(ox) / S, sc.y + float(oy) / S}, sc.z - bias);
+#pragma unroll
+        for (int oy = -1; oy <= 1; oy++) {
+            const f2 uv{sc.x + float(ox) / S, sc.y + float(oy) / S};
+            const float u = uv.x * S - 0.5f, w = uv.y * S - 0.5f;
+            const float fu = floorf(u), fw = floorf(w);
+            if (int32_t(fu) == i0 + ox && int32_t(fw) == j0 + oy) {
+                const float a = u - fu, b = w - fw;
+                const int k = 4 * (oy + 1) + (ox + 1);
+                auto bit = [&](int q) { return float((lit >> q) & 1u); };
+                shadow_sum += mixf(mixf(bit(k), bit(k + 1), a), mixf(bit(k + 4), bit(k + 5), a), b);
+            } else {
+                shadow_sum += shadow_tap(v, layer, uv, ref);
+            }
+        }
     shadow_sum /= 9.0f;
     if (sc.z > 1.0f) shadow_sum = 1.0f;
     return shadow_sum;
