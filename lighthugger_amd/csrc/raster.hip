@@ -263,11 +263,16 @@ __global__ void __launch_bounds__(256) k_raster_big(lh_scene s, VS vs, PassSet p
     const Queues qs = queues(v);
     const uint32_t count = min(v.counters[LH_CNT_BIG + ps.pass0], v.big_capacity);
     const int32_t tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
-    Tally<ST> tally;
+    // Statistics: each lane counts per pass of the set over all of the
+    // block's triangles, and the wave adds them to the view's counters once
+    // at the end (a flush per triangle also measured wrong depths on the
+    // large frame's sliver triangles with statistics on).
+    Tally<ST> tally[LH_NUM_CASCADES];
     for (uint32_t e = blockIdx.x; e < count; e += gridDim.x) {
         const BigTri b = qs.big[e];
         if (((b.ylo & kBigAlpha) != 0) != ALPHA) continue;  // block-uniform
-        const Pass p = make_pass(v, ps.pass0 + (uint32_t(b.xlo) >> 13));
+        const uint32_t pi = uint32_t(b.xlo) >> 13;            // block-uniform
+        const Pass p = make_pass(v, ps.pass0 + pi);
         const int32_t X0 = b.X[0], X1 = b.X[1], X2 = b.X[2], Y0 = b.Y[0], Y1 = b.Y[1], Y2 = b.Y[2];
         const int64_t area = edge64(X0, Y0, X1, Y1, X2, Y2);
         const float inv_area = 1.0f / float(area);
@@ -287,11 +292,16 @@ __global__ void __launch_bounds__(256) k_raster_big(lh_scene s, VS vs, PassSet p
                 if (!(e2 > 0 || (e2 == 0 && tl2))) continue;
                 if (ALPHA && !alpha_pass<KIND>(s, v, p, b.id, px, py)) continue;
                 const float l1 = float(e1) * inv_area, l2 = float(e2) * inv_area;
-                tally.hit(fragment<KIND>(p, px, py, (z0 + l1 * d10) + l2 * d20, b.id));
+                const bool issued = fragment<KIND>(p, px, py, (z0 + l1 * d10) + l2 * d20, b.id);
+#pragma unroll
+                for (uint32_t q = 0; q < LH_NUM_CASCADES; q++)
+                    if (q == pi) tally[q].hit(issued);
             }
         }
-        tally.flush(v, p.pass);
     }
+#pragma unroll
+    for (uint32_t q = 0; q < LH_NUM_CASCADES; q++)
+        if (q < ps.n) tally[q].flush(v, ps.pass0 + q);
 }
 
 // Batch plan: off[k + 1] = off[k] + surviving meshlets of segment k

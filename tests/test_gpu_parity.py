@@ -408,6 +408,35 @@ def test_raster_statistics_match_oracle():
     assert not plain["counters"][abi.CNT_FRAGMENTS:abi.CNT_FRAGMENTS + abi.NUM_PASSES].any()
 
 
+def test_raster_statistics_leave_the_large_frame_unchanged():
+    """The benchmark frame (its large-triangle queue holds ~33k slivers just
+    over the medium extent) with LH_VIEW_STATISTICS: two frames in a row are
+    bit-identical to the frame without statistics, and every pass counts at
+    least one covered sample per covered pixel / texel."""
+    import torch
+
+    from lighthugger_amd.lib import Renderer
+
+    scene = HostScene.preset(abi.PRESET_LARGE)
+    cam = preset_camera(abi.PRESET_LARGE)
+    plain = _gpu_frame(scene, cam, ViewConfig(cam.width, cam.height, 2048, big_capacity=1 << 21))
+    r = Renderer(scene, ViewConfig(cam.width, cam.height, 2048, big_capacity=1 << 21, statistics=True))
+    r.set_camera(cam)
+    for k in range(2):
+        r.frame()
+        torch.cuda.synchronize()
+        got = snapshot(r.view)
+        for key in ("visbuffer", "shadow", "radiance", "ldr"):
+            assert np.array_equal(plain[key], got[key]), (k, key)
+    c = got["counters"]
+    assert c[abi.CNT_BIG] > 1000  # the large-triangle raster ran
+    assert int(c[abi.CNT_FRAGMENTS]) >= int((got["visbuffer"] != np.uint64(abi.LH_VISBUFFER_CLEAR)).sum())
+    for q in range(abi.NUM_CASCADES):
+        covered = int((got["shadow"][q] != np.uint32(abi.LH_SHADOW_CLEAR)).sum())
+        print(f"cascade {q}: fragments {int(c[abi.CNT_FRAGMENTS + 1 + q])}, covered texels {covered}", flush=True)
+        assert int(c[abi.CNT_FRAGMENTS + 1 + q]) >= covered
+
+
 def _hiz_pair(scene, cam, cfg_kw):
     """The same frame with and without LH_VIEW_HIZ."""
     import torch
