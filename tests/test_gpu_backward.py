@@ -113,6 +113,14 @@ def test_backward_matches_fp64_adjoint(texture_format):
     got_r = {k: v.cpu().numpy().astype(np.float64) for k, v in rec.items()}
     for k in ("base_color", "roughness", "metallic", "sun_dir", "sun_intensity", "geometry"):
         assert _rel(got_r[k], got[k]) <= 1e-6, k
+    # a record without its valid bits (zero-filled, as a fresh or foreign
+    # buffer may be) is not trusted: every pixel recomputes the fixed quantities
+    d.frozen.zero_()
+    rec0 = d.raw_backward(p, torch.from_numpy(g).cuda(), recorded=True)
+    torch.cuda.synchronize()
+    got_0 = {k: v.cpu().numpy().astype(np.float64) for k, v in rec0.items()}
+    for k in ("base_color", "roughness", "metallic", "sun_dir", "sun_intensity", "geometry"):
+        assert _rel(got_0[k], got[k]) <= 1e-6, k
 
 
 def test_autograd_function_l2_loss():
