@@ -58,6 +58,8 @@ typedef void* lh_stream; /* hipStream_t */
 #define LH_CNT_ALPHA_DRAWS 48 /* + pass: surviving meshlets of alpha-clip mesh infos (with LH_VIEW_HIZ: before
                                  the occlusion test); zero skips the alpha-clip raster of the pass */
 #define LH_CNT_TILE_ITEMS 56   /* tiled main-view raster: work items (tile, run of binned meshlets) of the last launch */
+#define LH_CNT_TILE_NEXT 57     /* tiled main-view raster: work-item queue of the last launch (items handed
+                                  out beyond the grid's first) */
 #define LH_CNT_TILE_DIRECT 58  /* tiled main-view raster: meshlets left to the per-meshlet raster (beyond
                                   bin_capacity, or a screen rectangle over LH_TILE_MAX_RECT tiles) */
 #define LH_CNT_BACKWARD_TICKET 60 /* lh_resolve_backward: blocks done; zero between calls (the view reset

@@ -328,7 +328,10 @@ __global__ void __launch_bounds__(kScanThreads) k_bin_scan(OneView vs, uint32_t 
         }
         __syncthreads();
     }
-    if (threadIdx.x == 0) v.counters[LH_CNT_TILE_ITEMS] = min(carry[1], B.L.item_capacity);
+    if (threadIdx.x == 0) {
+        v.counters[LH_CNT_TILE_ITEMS] = min(carry[1], B.L.item_capacity);
+        v.counters[LH_CNT_TILE_NEXT] = 0;  // the tile raster's work queue starts empty
+    }
 }
 
 // One thread per binned draw: its entries at the tiles' cursors.  Per run of
@@ -549,7 +552,11 @@ __global__ void __launch_bounds__(kTileThreads) LH_WAVES_ATTR(8) k_tile_raster(O
     const uint32_t n_items = min(v.counters[LH_CNT_TILE_ITEMS], B.L.item_capacity);
     const int32_t W = int32_t(v.width), H = int32_t(v.height);
     Tally<ST> tally;
-    for (uint32_t it = blockIdx.x; it < n_items; it += gridDim.x) {
+    // Block b starts on item b; each further item comes from a queue, so a
+    // block that drew light items takes more of them instead of idling while
+    // the heavy ones finish.
+    __shared__ uint32_t next_item;
+    for (uint32_t it = blockIdx.x; it < n_items;) {
         const TileItem item = B.items[it];
         const int32_t ox = int32_t(item.tile % B.L.tiles_x) * int32_t(kTile);
         const int32_t oy = int32_t(item.tile / B.L.tiles_x) * int32_t(kTile);
@@ -631,7 +638,9 @@ __global__ void __launch_bounds__(kTileThreads) LH_WAVES_ATTR(8) k_tile_raster(O
             else
                 atomicMin(dst, (unsigned long long)key);
         }
-        __syncthreads();
+        if (threadIdx.x == 0) next_item = gridDim.x + atomicAdd(&v.counters[LH_CNT_TILE_NEXT], 1u);
+        __syncthreads();  // the item's LDS tile is free; next_item is written
+        it = next_item;   // rewritten only after the next item's barriers
     }
 }
 
