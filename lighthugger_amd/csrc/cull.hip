@@ -116,6 +116,29 @@ LH_DEV bool prefix_search(const uint64_t* sec, uint32_t n, uint32_t target, Pref
     return true;
 }
 
+// Block-aggregated append for 256-thread blocks (every thread calls it, the
+// same number of times): one device atomic per block instead of one per
+// wave, since the appends of a pass all go to one counter and same-address
+// atomics serialize.  Returns this thread's slot when pred holds.
+constexpr uint32_t kCullThreads = 256;
+LH_DEV uint32_t block_append(uint32_t* counter, bool pred, uint32_t* counts /* LDS [kCullThreads / 64 + 1] */) {
+    const uint64_t m = ballot(pred);
+    const uint32_t wave = threadIdx.x >> 6;
+    if (lane_id() == 0) counts[wave] = uint32_t(__popcll(m));
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t total = 0;
+        for (uint32_t w = 0; w < kCullThreads / 64; w++) total += counts[w];
+        counts[kCullThreads / 64] = total ? atomicAdd(counter, total) : 0u;
+    }
+    __syncthreads();
+    uint32_t slot = counts[kCullThreads / 64];
+    for (uint32_t w = 0; w < wave; w++) slot += counts[w];
+    slot += lanes_below(m);
+    __syncthreads();  // counts is reused by the next call
+    return slot;
+}
+
 }  // namespace
 
 template <class VS>
@@ -209,7 +232,8 @@ __global__ void __launch_bounds__(256) k_fit_cull_instances_shadow(lh_scene s, V
 // unbounded screen footprints go to draws[0] (phase-1 occluders), small ones
 // to the candidate list staged in draws[1].
 template <class VS>
-__global__ void __launch_bounds__(256) k_cull_meshlets(lh_scene s, VS vs, uint32_t pass0) {
+__global__ void __launch_bounds__(kCullThreads) k_cull_meshlets(lh_scene s, VS vs, uint32_t pass0) {
+    __shared__ uint32_t counts[kCullThreads / 64 + 1];
     const lh_view& v = vs.get(blockIdx.z);
     const uint32_t pass = pass0 + blockIdx.y;
     const Uniforms& u = *v.uniforms;
@@ -254,7 +278,7 @@ __global__ void __launch_bounds__(256) k_cull_meshlets(lh_scene s, VS vs, uint32
             }
         }
         wave_append(&v.counters[LH_CNT_ALPHA_DRAWS + pass], (survive || candidate) && alpha);
-        uint32_t slot = wave_append(&v.counters[LH_CNT_DRAWS + pass], survive);
+        uint32_t slot = block_append(&v.counters[LH_CNT_DRAWS + pass], survive, counts);
         if (survive) {
             if (slot < v.draw_capacity) {
                 MeshletReference r;
