@@ -28,7 +28,8 @@
 //                      (both appended with one atomic per wave, tagged with
 //                      the pass).
 //   k_raster_clip      clips queued triangles and queues the fan pieces.
-//   k_raster_big       one 256-thread block per queued triangle, 16x16 tiles.
+//   k_raster_big       one 256-thread block per queued triangle, 256-pixel
+//                      chunks shaped to the triangle's box.
 #include "raster.h"
 
 using namespace lhd;
@@ -262,7 +263,6 @@ __global__ void __launch_bounds__(256) k_raster_big(lh_scene s, VS vs, PassSet p
     }
     const Queues qs = queues(v);
     const uint32_t count = min(v.counters[LH_CNT_BIG + ps.pass0], v.big_capacity);
-    const int32_t tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
     // Statistics: each lane counts per pass of the set over all of the
     // block's triangles, and the wave adds them to the view's counters once
     // at the end (a flush per triangle also measured wrong depths on the
@@ -279,8 +279,18 @@ __global__ void __launch_bounds__(256) k_raster_big(lh_scene s, VS vs, PassSet p
         const float z0 = b.z[0], d10 = b.z[1] - b.z[0], d20 = b.z[2] - b.z[0];
         const bool tl0 = top_left(X1, Y1, X2, Y2), tl1 = top_left(X2, Y2, X0, Y0), tl2 = top_left(X0, Y0, X1, Y1);
         const int32_t xlo = b.xlo & 0x1FFF, xhi = b.xhi, ylo = b.ylo & ~kBigAlpha, yhi = b.yhi;
-        for (int32_t by = ylo; by <= yhi; by += 16) {
-            for (int32_t bx = xlo; bx <= xhi; bx += 16) {
+        // The block's 256 threads cover chunks of 2^cs x 2^(8 - cs) pixels
+        // shaped to the box: a flat sliver (most triangles just over the
+        // medium extent) takes one 64 x 4 chunk instead of two 16 x 16.
+        const int32_t bw = xhi - xlo + 1, bh = yhi - ylo + 1;
+        int32_t cs = 4;  // log2 of the chunk width
+        if (bh < bw)
+            cs = 8 - min(4, 32 - __clz(max(bh - 1, 0)));  // height rounded up to a power of two, at most 16
+        else if (bw < bh)
+            cs = min(4, 32 - __clz(max(bw - 1, 0)));
+        const int32_t tx = int32_t(threadIdx.x) & ((1 << cs) - 1), ty = int32_t(threadIdx.x) >> cs;
+        for (int32_t by = ylo; by <= yhi; by += 256 >> cs) {
+            for (int32_t bx = xlo; bx <= xhi; bx += 1 << cs) {
                 const int32_t px = bx + tx, py = by + ty;
                 if (px > xhi || py > yhi) continue;
                 const int32_t Px = px * 256 + 128, Py = py * 256 + 128;
