@@ -28,8 +28,8 @@
 //                      (both appended with one atomic per wave, tagged with
 //                      the pass).
 //   k_raster_clip      clips queued triangles and queues the fan pieces.
-//   k_raster_big       one 256-thread block per queued triangle, 256-pixel
-//                      chunks shaped to the triangle's box.
+//   k_raster_big       queued triangles, one wave each (whole block for
+//                      boxes over kWaveBoxPx pixels).
 #include "raster.h"
 
 using namespace lhd;
@@ -253,6 +253,58 @@ __global__ void __launch_bounds__(64) k_raster_clip(lh_scene s, VS vs, PassSet p
 // ALPHA selects the entries of alpha-clip geometry (alpha test per sample),
 // so the opaque instantiation keeps its small register footprint; the alpha
 // one returns at once when the view's culls counted no alpha-clip draw.
+// Triangles of the large queue whose pixel box holds at most this many pixels
+// are rasterized by one wave alone; larger ones by the whole block.
+constexpr int32_t kWaveBoxPx = 4096;
+
+// Rasterizes the pixels of one queued triangle's box that fall on lanes
+// first, first + stride, ... of the box's row-major pixel order.  Coverage
+// and depth of each pixel are evaluated exactly as in the other paths
+// (64-bit edge functions at the pixel centre, top-left rule).
+template <int KIND, bool ST, bool ALPHA>
+LH_DEV void raster_box(const lh_scene& s, const lh_view& v, const PassSet& ps, const BigTri& b, int32_t first,
+                       int32_t stride, Tally<ST>* tally) {
+    const uint32_t pi = uint32_t(b.xlo) >> 13;
+    const Pass p = make_pass(v, ps.pass0 + pi);
+    const int32_t X0 = b.X[0], X1 = b.X[1], X2 = b.X[2], Y0 = b.Y[0], Y1 = b.Y[1], Y2 = b.Y[2];
+    const int64_t area = edge64(X0, Y0, X1, Y1, X2, Y2);
+    const float inv_area = 1.0f / float(area);
+    const float z0 = b.z[0], d10 = b.z[1] - b.z[0], d20 = b.z[2] - b.z[0];
+    const bool tl0 = top_left(X1, Y1, X2, Y2), tl1 = top_left(X2, Y2, X0, Y0), tl2 = top_left(X0, Y0, X1, Y1);
+    const int32_t xlo = b.xlo & 0x1FFF, xhi = b.xhi, ylo = b.ylo & ~kBigAlpha, yhi = b.yhi;
+    const int32_t bw = xhi - xlo + 1, n = bw * (yhi - ylo + 1);
+    if (first >= n) return;
+    // Step by `stride` pixels in row-major order: sy rows and sx columns,
+    // plus one row when the column passes the box's right edge.
+    const int32_t sy = stride / bw, sx = stride - sy * bw;
+    int32_t py = ylo + first / bw, px = xlo + (first - (first / bw) * bw);
+    for (int32_t i = first; i < n; i += stride) {
+        const int32_t Px = px * 256 + 128, Py = py * 256 + 128;
+        const int64_t e0 = edge64(X1, Y1, X2, Y2, Px, Py);
+        const int64_t e1 = edge64(X2, Y2, X0, Y0, Px, Py);
+        const int64_t e2 = edge64(X0, Y0, X1, Y1, Px, Py);
+        const bool in = (e0 > 0 || (e0 == 0 && tl0)) && (e1 > 0 || (e1 == 0 && tl1)) && (e2 > 0 || (e2 == 0 && tl2));
+        if (in && (!ALPHA || alpha_pass<KIND>(s, v, p, b.id, px, py))) {
+            const float l1 = float(e1) * inv_area, l2 = float(e2) * inv_area;
+            const bool issued = fragment<KIND>(p, px, py, (z0 + l1 * d10) + l2 * d20, b.id);
+#pragma unroll
+            for (uint32_t q = 0; q < LH_NUM_CASCADES; q++)
+                if (q == pi) tally[q].hit(issued);
+        }
+        px += sx;
+        py += sy;
+        if (px > xhi) {
+            px -= bw;
+            py++;
+        }
+    }
+}
+
+// Each round the block takes four queue entries, one per wave.  A wave
+// rasterizes its own triangle when the box is at most kWaveBoxPx pixels (most
+// of the queue: slivers just over the medium extent), so a 54-pixel box costs
+// one wave pass instead of a 256-thread block; the larger boxes of the round
+// are rasterized by all four waves together first.
 template <int KIND, class VS, bool ST, bool ALPHA>
 __global__ void __launch_bounds__(256) k_raster_big(lh_scene s, VS vs, PassSet ps) {
     const lh_view& v = vs.get(blockIdx.z);
@@ -263,51 +315,35 @@ __global__ void __launch_bounds__(256) k_raster_big(lh_scene s, VS vs, PassSet p
     }
     const Queues qs = queues(v);
     const uint32_t count = min(v.counters[LH_CNT_BIG + ps.pass0], v.big_capacity);
+    const uint32_t wave = threadIdx.x >> 6;
+    // wide[r & 1][w]: wave w's triangle of round r goes to the whole block.
+    // Two buffers, because a wave may write round r + 1's flag while another
+    // still reads round r's (no wave gets two rounds ahead: each round has a
+    // barrier).
+    __shared__ uint32_t wide[2][4];
     // Statistics: each lane counts per pass of the set over all of the
     // block's triangles, and the wave adds them to the view's counters once
     // at the end (a flush per triangle also measured wrong depths on the
     // large frame's sliver triangles with statistics on).
     Tally<ST> tally[LH_NUM_CASCADES];
-    for (uint32_t e = blockIdx.x; e < count; e += gridDim.x) {
-        const BigTri b = qs.big[e];
-        if (((b.ylo & kBigAlpha) != 0) != ALPHA) continue;  // block-uniform
-        const uint32_t pi = uint32_t(b.xlo) >> 13;            // block-uniform
-        const Pass p = make_pass(v, ps.pass0 + pi);
-        const int32_t X0 = b.X[0], X1 = b.X[1], X2 = b.X[2], Y0 = b.Y[0], Y1 = b.Y[1], Y2 = b.Y[2];
-        const int64_t area = edge64(X0, Y0, X1, Y1, X2, Y2);
-        const float inv_area = 1.0f / float(area);
-        const float z0 = b.z[0], d10 = b.z[1] - b.z[0], d20 = b.z[2] - b.z[0];
-        const bool tl0 = top_left(X1, Y1, X2, Y2), tl1 = top_left(X2, Y2, X0, Y0), tl2 = top_left(X0, Y0, X1, Y1);
-        const int32_t xlo = b.xlo & 0x1FFF, xhi = b.xhi, ylo = b.ylo & ~kBigAlpha, yhi = b.yhi;
-        // The block's 256 threads cover chunks of 2^cs x 2^(8 - cs) pixels
-        // shaped to the box: a flat sliver (most triangles just over the
-        // medium extent) takes one 64 x 4 chunk instead of two 16 x 16.
-        const int32_t bw = xhi - xlo + 1, bh = yhi - ylo + 1;
-        int32_t cs = 4;  // log2 of the chunk width
-        if (bh < bw)
-            cs = 8 - min(4, 32 - __clz(max(bh - 1, 0)));  // height rounded up to a power of two, at most 16
-        else if (bw < bh)
-            cs = min(4, 32 - __clz(max(bw - 1, 0)));
-        const int32_t tx = int32_t(threadIdx.x) & ((1 << cs) - 1), ty = int32_t(threadIdx.x) >> cs;
-        for (int32_t by = ylo; by <= yhi; by += 256 >> cs) {
-            for (int32_t bx = xlo; bx <= xhi; bx += 1 << cs) {
-                const int32_t px = bx + tx, py = by + ty;
-                if (px > xhi || py > yhi) continue;
-                const int32_t Px = px * 256 + 128, Py = py * 256 + 128;
-                const int64_t e0 = edge64(X1, Y1, X2, Y2, Px, Py);
-                const int64_t e1 = edge64(X2, Y2, X0, Y0, Px, Py);
-                const int64_t e2 = edge64(X0, Y0, X1, Y1, Px, Py);
-                if (!(e0 > 0 || (e0 == 0 && tl0))) continue;
-                if (!(e1 > 0 || (e1 == 0 && tl1))) continue;
-                if (!(e2 > 0 || (e2 == 0 && tl2))) continue;
-                if (ALPHA && !alpha_pass<KIND>(s, v, p, b.id, px, py)) continue;
-                const float l1 = float(e1) * inv_area, l2 = float(e2) * inv_area;
-                const bool issued = fragment<KIND>(p, px, py, (z0 + l1 * d10) + l2 * d20, b.id);
-#pragma unroll
-                for (uint32_t q = 0; q < LH_NUM_CASCADES; q++)
-                    if (q == pi) tally[q].hit(issued);
+    uint32_t round = 0;
+    for (uint32_t e0 = blockIdx.x * 4; e0 < count; e0 += gridDim.x * 4, round ^= 1) {  // block-uniform
+        const uint32_t e = e0 + wave;
+        bool own = false, shared = false;
+        BigTri b;
+        if (e < count) {
+            b = qs.big[e];
+            if (((b.ylo & kBigAlpha) != 0) == ALPHA) {
+                const int32_t bw = b.xhi - (b.xlo & 0x1FFF) + 1, bh = b.yhi - (b.ylo & ~kBigAlpha) + 1;
+                own = bw * bh <= kWaveBoxPx;
+                shared = !own;
             }
         }
+        if (lane_id() == 0) wide[round][wave] = shared ? 1u : 0u;
+        __syncthreads();
+        for (uint32_t w = 0; w < 4; w++)
+            if (wide[round][w]) raster_box<KIND, ST, ALPHA>(s, v, ps, qs.big[e0 + w], threadIdx.x, 256, tally);
+        if (own) raster_box<KIND, ST, ALPHA>(s, v, ps, b, int32_t(lane_id()), 64, tally);
     }
 #pragma unroll
     for (uint32_t q = 0; q < LH_NUM_CASCADES; q++)
