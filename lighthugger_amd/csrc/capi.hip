@@ -40,14 +40,16 @@ hipError_t raster_main(const lh_scene& s, const lhd::OneView& vs, const Tuning& 
 hipError_t record_frame(const lh_scene& s, const lh_view& v, hipStream_t st) {
     const lhd::OneView vs{v};
     const uint64_t px = uint64_t(v.width) * v.height;
+    // The attachment clears (src/rendering.cpp:206-211, :384) ride on the
+    // meshlet culls: the view's cull clears the visibility buffer, the
+    // cascades' cull the shadow maps, each before the raster that writes them.
     hipError_t e = view_reset(vs, st);
-    if (e == hipSuccess) e = clear_targets(vs, std::max<uint64_t>(px, uint64_t(v.shadow_size) * v.shadow_size), st);
     if (e == hipSuccess) e = cull_instances(s, vs, false, st);
-    if (e == hipSuccess) e = cull_meshlets(s, vs, 0, 1, tuning_for(v).cull_blocks, st);
+    if (e == hipSuccess) e = cull_meshlets(s, vs, 0, 1, tuning_for(v).cull_blocks, st, kClearVisbuffer);
     if (e == hipSuccess) e = raster_main(s, vs, tuning_for(v), st);
     if (e == hipSuccess) e = depth_bounds(vs, px, st);
     if (e == hipSuccess) e = fit_cull_instances_shadow(s, vs, st);
-    if (e == hipSuccess) e = cull_meshlets(s, vs, 1, LH_NUM_CASCADES, tuning_for(v).cull_blocks, st);
+    if (e == hipSuccess) e = cull_meshlets(s, vs, 1, LH_NUM_CASCADES, tuning_for(v).cull_blocks, st, kClearShadowMaps);
     if (e == hipSuccess) e = raster(s, vs, 1, LH_NUM_CASCADES, tuning_for(v).raster, st);
     if (e == hipSuccess) e = resolve(s, vs, 0, lh_resolve_params{}, st);
     return e;

@@ -153,22 +153,29 @@ __global__ void __launch_bounds__(64) k_view_reset(VS vs) {
     }
 }
 
-// Clears the visibility buffer (two keys per 16-byte store) and the shadow
-// maps (four texels per store) of view blockIdx.z with plain vector stores.
+// Clears the visibility buffer (two keys per 16-byte store, when VIS) and the
+// shadow maps (four texels per store, when SHADOW) of view v with plain
+// vector stores: thread t0 of `stride` threads takes stores t0, t0 + stride, ...
+LH_DEV void clear_part(const lh_view& v, bool vis, bool shadow, uint64_t t0, uint64_t stride) {
+    if (vis) {
+        const uint64_t nv = uint64_t(v.width) * v.height;
+        const ulonglong2 kv = make_ulonglong2(LH_VISBUFFER_CLEAR, LH_VISBUFFER_CLEAR);
+        for (uint64_t i = t0; 2 * i + 1 < nv; i += stride) reinterpret_cast<ulonglong2*>(v.visbuffer)[i] = kv;
+        if (t0 == 0 && (nv & 1)) v.visbuffer[nv - 1] = LH_VISBUFFER_CLEAR;
+    }
+    if (shadow && v.shadow_maps) {
+        const uint64_t ns = uint64_t(LH_NUM_CASCADES) * v.shadow_size * v.shadow_size;
+        const uint4 ks = make_uint4(LH_SHADOW_CLEAR, LH_SHADOW_CLEAR, LH_SHADOW_CLEAR, LH_SHADOW_CLEAR);
+        for (uint64_t i = t0; 4 * i + 3 < ns; i += stride) reinterpret_cast<uint4*>(v.shadow_maps)[i] = ks;
+        if (t0 == 0)
+            for (uint64_t i = ns & ~uint64_t(3); i < ns; i++) v.shadow_maps[i] = LH_SHADOW_CLEAR;
+    }
+}
+
 template <class VS>
 __global__ void __launch_bounds__(256) k_clear_targets(VS vs) {
-    const lh_view& v = vs.get(blockIdx.z);
-    const uint64_t nv = uint64_t(v.width) * v.height;
-    const uint64_t ns = v.shadow_maps ? uint64_t(LH_NUM_CASCADES) * v.shadow_size * v.shadow_size : 0;
-    const uint64_t stride = uint64_t(gridDim.x) * blockDim.x;
-    const uint64_t t0 = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    const ulonglong2 kv = make_ulonglong2(LH_VISBUFFER_CLEAR, LH_VISBUFFER_CLEAR);
-    for (uint64_t i = t0; 2 * i + 1 < nv; i += stride) reinterpret_cast<ulonglong2*>(v.visbuffer)[i] = kv;
-    if (t0 == 0 && (nv & 1)) v.visbuffer[nv - 1] = LH_VISBUFFER_CLEAR;
-    const uint4 ks = make_uint4(LH_SHADOW_CLEAR, LH_SHADOW_CLEAR, LH_SHADOW_CLEAR, LH_SHADOW_CLEAR);
-    for (uint64_t i = t0; 4 * i + 3 < ns; i += stride) reinterpret_cast<uint4*>(v.shadow_maps)[i] = ks;
-    if (t0 == 0)
-        for (uint64_t i = ns & ~uint64_t(3); i < ns; i++) v.shadow_maps[i] = LH_SHADOW_CLEAR;
+    clear_part(vs.get(blockIdx.z), true, true, uint64_t(blockIdx.x) * blockDim.x + threadIdx.x,
+               uint64_t(gridDim.x) * blockDim.x);
 }
 
 // Main view (shadow = false, sector 0) or cascade blockIdx.y (sector 1 + c).
@@ -231,8 +238,11 @@ __global__ void __launch_bounds__(256) k_fit_cull_instances_shadow(lh_scene s, V
 // With LH_VIEW_HIZ (single view) the main-view survivors are split: large or
 // unbounded screen footprints go to draws[0] (phase-1 occluders), small ones
 // to the candidate list staged in draws[1].
+// `clear` (LH_CLEAR_*): render targets of the view that the grid clears after
+// its culling, as a clear kernel launched before the cull would; the stores
+// follow each block's last load, so they do not hold up the cull's loads.
 template <class VS>
-__global__ void __launch_bounds__(kCullThreads) k_cull_meshlets(lh_scene s, VS vs, uint32_t pass0) {
+__global__ void __launch_bounds__(kCullThreads) k_cull_meshlets(lh_scene s, VS vs, uint32_t pass0, uint32_t clear) {
     __shared__ uint32_t counts[kCullThreads / 64 + 1];
     const lh_view& v = vs.get(blockIdx.z);
     const uint32_t pass = pass0 + blockIdx.y;
@@ -303,6 +313,11 @@ __global__ void __launch_bounds__(kCullThreads) k_cull_meshlets(lh_scene s, VS v
             }
         }
     }
+    if (clear != 0) {
+        const uint64_t b = uint64_t(blockIdx.y) * gridDim.x + blockIdx.x;
+        clear_part(v, (clear & lhk::kClearVisbuffer) != 0, (clear & lhk::kClearShadowMaps) != 0,
+                   b * blockDim.x + threadIdx.x, uint64_t(gridDim.x) * gridDim.y * blockDim.x);
+    }
 }
 
 // ---------------------------------------------------------------- launchers
@@ -349,10 +364,10 @@ hipError_t fit_cull_instances_shadow(const lh_scene& s, const VS& vs, hipStream_
 
 template <class VS>
 hipError_t cull_meshlets(const lh_scene& s, const VS& vs, uint32_t pass0, uint32_t npasses, uint32_t blocks,
-                         hipStream_t st) {
+                         hipStream_t st, uint32_t clear) {
     if (vs.count() > 1) blocks = std::max(blocks / 8, 32u);
     mark(pass0 == 0 ? "cull_meshlets_view" : "cull_meshlets_shadow", st);
-    hipLaunchKernelGGL(k_cull_meshlets<VS>, dim3(blocks, npasses, vs.count()), dim3(256), 0, st, s, vs, pass0);
+    hipLaunchKernelGGL(k_cull_meshlets<VS>, dim3(blocks, npasses, vs.count()), dim3(256), 0, st, s, vs, pass0, clear);
     return hipGetLastError();
 }
 
@@ -361,7 +376,7 @@ hipError_t cull_meshlets(const lh_scene& s, const VS& vs, uint32_t pass0, uint32
     template hipError_t clear_targets<VS>(const VS&, uint64_t, hipStream_t);                                       \
     template hipError_t cull_instances<VS>(const lh_scene&, const VS&, bool, hipStream_t);                        \
     template hipError_t fit_cull_instances_shadow<VS>(const lh_scene&, const VS&, hipStream_t);                   \
-    template hipError_t cull_meshlets<VS>(const lh_scene&, const VS&, uint32_t, uint32_t, uint32_t, hipStream_t);
+    template hipError_t cull_meshlets<VS>(const lh_scene&, const VS&, uint32_t, uint32_t, uint32_t, hipStream_t, uint32_t);
 LH_INSTANTIATE(lhd::OneView)
 LH_INSTANTIATE(lhd::ViewTable)
 #undef LH_INSTANTIATE

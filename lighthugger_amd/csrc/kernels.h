@@ -58,9 +58,13 @@ hipError_t cull_instances(const lh_scene& s, const VS& vs, bool shadow, hipStrea
 // generate_shadow_matrices + cull_instances_shadows in one launch.
 template <class VS>
 hipError_t fit_cull_instances_shadow(const lh_scene& s, const VS& vs, hipStream_t st);
+// Render targets a meshlet-cull launch clears after its culling (the frame's
+// clears, moved into the latency-bound cull launches that precede the
+// rasters writing those targets).
+constexpr uint32_t kClearVisbuffer = 1, kClearShadowMaps = 2;
 template <class VS>
 hipError_t cull_meshlets(const lh_scene& s, const VS& vs, uint32_t pass0, uint32_t npasses, uint32_t blocks,
-                         hipStream_t st);
+                         hipStream_t st, uint32_t clear = 0);
 // Rasterizes passes [pass0, pass0 + npasses) in one set of launches (the main
 // view alone, or any run of cascades).
 template <class VS>

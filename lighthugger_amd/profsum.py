@@ -39,7 +39,6 @@ FRAME_KERNELS = {
     "raster_clip_view": "k_raster_clip<0, lhd::OneView>",
     "raster_clip_shadow": "k_raster_clip<1, lhd::OneView>",
     "resolve": "k_resolve",
-    "clear_targets": "k_clear_targets<lhd::OneView>",
     "depth_bounds": "k_depth_bounds<lhd::OneView>",
     "fit_cull_instances_shadow": "k_fit_cull_instances_shadow<lhd::OneView>",
     "copy": "k_copy",
