@@ -117,7 +117,14 @@ LH_DEV bool finite4(f4 v) { return isfinite(v.x) && isfinite(v.y) && isfinite(v.
 
 LH_DEV bool project(f4 c, uint32_t W, uint32_t H, SV& o) {
     if (!(c.w > 0.0f)) return false;
-    float xn = c.x / c.w, yn = c.y / c.w, zn = c.z / c.w;
+    // x / 1 is x exactly: the orthographic cascades (w == 1 at every vertex)
+    // skip the three correctly rounded divisions.
+    float xn = c.x, yn = c.y, zn = c.z;
+    if (c.w != 1.0f) {
+        xn = c.x / c.w;
+        yn = c.y / c.w;
+        zn = c.z / c.w;
+    }
     float hw = 0.5f * float(W), hh = 0.5f * float(H);
     float xf = xn * hw + hw, yf = yn * hh + hh;
     o.X = int32_t(rintf(xf * 256.0f));
@@ -136,28 +143,29 @@ LH_DEV bool top_left(int32_t ax, int32_t ay, int32_t bx, int32_t by) {
 
 // Culls by facing (alpha-clip geometry is double-sided: only zero area is
 // culled, src/pipelines.cpp:22-25), orients to positive area and computes the
-// pixel bbox.
+// pixel bbox.  Every output is written on every path (the bbox first: it does
+// not depend on the vertex order).
 template <int KIND>
 LH_DEV bool setup(SV& v0, SV& v1, SV& v2, uint32_t W, uint32_t H, int64_t& area, int32_t& xlo, int32_t& xhi,
                   int32_t& ylo, int32_t& yhi, int32_t& extx, int32_t& exty, bool two_sided = false,
                   bool* swapped = nullptr) {
-    area = edge64(v0.X, v0.Y, v1.X, v1.Y, v2.X, v2.Y);
-    if (two_sided ? area == 0 : (KIND == 0 ? area >= 0 : area <= 0)) return false;
-    if (swapped) *swapped = area < 0;
-    if (area < 0) {
-        SV t = v1;
-        v1 = v2;
-        v2 = t;
-        area = -area;
-    }
-    int32_t minX = min(v0.X, min(v1.X, v2.X)), maxX = max(v0.X, max(v1.X, v2.X));
-    int32_t minY = min(v0.Y, min(v1.Y, v2.Y)), maxY = max(v0.Y, max(v1.Y, v2.Y));
+    const int32_t minX = min(v0.X, min(v1.X, v2.X)), maxX = max(v0.X, max(v1.X, v2.X));
+    const int32_t minY = min(v0.Y, min(v1.Y, v2.Y)), maxY = max(v0.Y, max(v1.Y, v2.Y));
     xlo = max((minX + 127) >> 8, 0);
     xhi = min((maxX - 128) >> 8, int32_t(W) - 1);
     ylo = max((minY + 127) >> 8, 0);
     yhi = min((maxY - 128) >> 8, int32_t(H) - 1);
     extx = maxX - minX;
     exty = maxY - minY;
+    area = edge64(v0.X, v0.Y, v1.X, v1.Y, v2.X, v2.Y);
+    if (swapped) *swapped = area < 0;
+    if (two_sided ? area == 0 : (KIND == 0 ? area >= 0 : area <= 0)) return false;
+    if (area < 0) {
+        SV t = v1;
+        v1 = v2;
+        v2 = t;
+        area = -area;
+    }
     return xlo <= xhi && ylo <= yhi;
 }
 

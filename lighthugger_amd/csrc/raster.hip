@@ -82,45 +82,40 @@ __global__ void __launch_bounds__(64) LH_RASTER_OCCUPANCY k_raster_meshlets(lh_s
             f3 w = mat4_point(in.transform, load_position(mi, load_index(mi, ml.index_offset + lane)));
             f4 c = mat4_mul_vec4(M, f4{w.x, w.y, w.z, 1.0f});
             uint32_t oc = finite4(c) ? outcode(c) : kBadVertex;
-            if (oc == 0) {
-                SV sv;
-                if (project(c, p.W, p.H, sv)) {
-                    sX[lane] = sv.X;
-                    sY[lane] = sv.Y;
-                    sZ[lane] = sv.z;
-                } else {
-                    oc = kBadVertex;
-                }
-            }
+            // clipped and invalid vertices store zeros, so that the triangle
+            // setup below may read any corner
+            SV sv{0, 0, 0.0f};
+            if (oc == 0 && !project(c, p.W, p.H, sv)) oc = kBadVertex;
+            sX[lane] = sv.X;
+            sY[lane] = sv.Y;
+            sZ[lane] = sv.z;
             sO[lane] = oc;
         }
         __syncthreads();
         for (uint32_t round = 0; round < 2; round++) {
             const uint32_t t = lane + 64 * round;
             if (__ballot(t < ntri) == 0) break;
-            // 0 none, 1 clip queue, 2 big queue, 3 in-lane, 4 wave-cooperative
-            uint32_t cls = 0;
-            SV v0{}, v1{}, v2{};
-            int64_t area = 0;
-            int32_t xlo = 0, xhi = 0, ylo = 0, yhi = 0, ex = 0, ey = 0;
-            if (t < ntri) {
-                const uint32_t packed = corners[round];
-                uint32_t i0 = packed & 255u, i1 = (packed >> 8) & 255u, i2 = packed >> 16;
-                uint32_t o0 = sO[i0], o1 = sO[i1], o2 = sO[i2];
-                if (((o0 | o1 | o2) & kBadVertex) == 0 && (o0 & o1 & o2) == 0) {
-                    if (o0 | o1 | o2) {
-                        cls = clip_queued ? 0u : 1u;  // the tiled raster's binning queued it already
-                    } else {
-                        v0 = SV{sX[i0], sY[i0], sZ[i0]};
-                        v1 = SV{sX[i1], sY[i1], sZ[i1]};
-                        v2 = SV{sX[i2], sY[i2], sZ[i2]};
-                        if (setup<KIND>(v0, v1, v2, p.W, p.H, area, xlo, xhi, ylo, yhi, ex, ey, ALPHA)) {
-                            const int32_t e = max(ex, ey);
-                            cls = e <= ps.small_extent ? 3u : e <= ps.medium_extent ? 4u : 2u;
-                        }
-                    }
-                }
-            }
+            // Every lane sets up its triangle (lanes past the meshlet read
+            // corner 0 three times: zero area) and the class is computed
+            // without nested branches, whose exits would each re-zero the
+            // setup's registers.
+            const uint32_t packed = corners[round];
+            const uint32_t i0 = packed & 255u, i1 = (packed >> 8) & 255u, i2 = packed >> 16;
+            const uint32_t o0 = sO[i0], o1 = sO[i1], o2 = sO[i2];
+            SV v0{sX[i0], sY[i0], sZ[i0]}, v1{sX[i1], sY[i1], sZ[i1]}, v2{sX[i2], sY[i2], sZ[i2]};
+            int64_t area;
+            int32_t xlo, xhi, ylo, yhi, ex, ey;
+            const bool set = setup<KIND>(v0, v1, v2, p.W, p.H, area, xlo, xhi, ylo, yhi, ex, ey, ALPHA);
+            const bool drawn = t < ntri && ((o0 | o1 | o2) & kBadVertex) == 0 && (o0 & o1 & o2) == 0;
+            const int32_t e = max(ex, ey);
+            // 0 none, 1 clip queue (unless the tiled raster's binning queued
+            // it already), 2 big queue, 3 in-lane, 4 wave-cooperative
+            const uint32_t cls = !drawn                 ? 0u
+                                 : (o0 | o1 | o2) != 0  ? (clip_queued ? 0u : 1u)
+                                 : !set                 ? 0u
+                                 : e <= ps.small_extent ? 3u
+                                 : e <= ps.medium_extent ? 4u
+                                                         : 2u;
             const uint32_t id = (mid << LH_KEY_TRI_BITS) | t;
             uint32_t cslot = wave_append(clip_counter, cls == 1);
             if (cls == 1) {

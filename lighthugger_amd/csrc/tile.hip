@@ -144,38 +144,31 @@ __global__ void __launch_bounds__(64) LH_RASTER_OCCUPANCY k_bin_meshlets(lh_scen
         for (uint32_t round = 0; round < 2; round++) {
             const uint32_t t = lane + 64 * round;
             if (__ballot(t < ntri) == 0) break;
-            bool clip = false, front = false, big = false;
-            SV bv0{}, bv1{}, bv2{};
-            int32_t bx0 = 0, bx1 = 0, by0 = 0, by1 = 0;
+            // Every lane sets up its triangle (lanes past the meshlet read
+            // corner 0 three times: zero area; clipped corners read as zero)
+            // and the outcome is flat predicates, not nested branches whose
+            // exits would each re-zero the setup's registers.
             uint32_t packed = corners[round];
-            if (t < ntri) {
-                uint32_t i0 = packed & 255u, i1 = (packed >> 8) & 255u, i2 = packed >> 16;
-                uint32_t o0 = sO[i0], o1 = sO[i1], o2 = sO[i2];
-                if (((o0 | o1 | o2) & kBadVertex) == 0 && (o0 & o1 & o2) == 0) {
-                    if (o0 | o1 | o2) {
-                        clip = true;
-                    } else {
-                        SV v0{sX[i0], sY[i0], sZ[i0]}, v1{sX[i1], sY[i1], sZ[i1]}, v2{sX[i2], sY[i2], sZ[i2]};
-                        int64_t area;
-                        int32_t xlo, xhi, ylo, yhi, ex, ey;
-                        bool swapped = false;
-                        if (setup<KIND>(v0, v1, v2, p.W, p.H, area, xlo, xhi, ylo, yhi, ex, ey, false, &swapped)) {
-                            if (max(ex, ey) > ps.medium_extent) {
-                                big = true;  // spread over many blocks by the large-triangle raster
-                                bv0 = v0, bv1 = v1, bv2 = v2;
-                                bx0 = xlo, bx1 = xhi, by0 = ylo, by1 = yhi;
-                            } else {
-                                front = true;
-                                if (swapped) packed = i0 | (i2 << 8) | (i1 << 16);
-                                xlo_m = min(xlo_m, xlo);
-                                ylo_m = min(ylo_m, ylo);
-                                xhi_m = max(xhi_m, xhi);
-                                yhi_m = max(yhi_m, yhi);
-                                work += uint32_t((xhi - xlo + 1) * (yhi - ylo + 1));
-                            }
-                        }
-                    }
-                }
+            const uint32_t i0 = packed & 255u, i1 = (packed >> 8) & 255u, i2 = packed >> 16;
+            const uint32_t o0 = sO[i0], o1 = sO[i1], o2 = sO[i2];
+            SV v0{sX[i0], sY[i0], sZ[i0]}, v1{sX[i1], sY[i1], sZ[i1]}, v2{sX[i2], sY[i2], sZ[i2]};
+            int64_t area;
+            int32_t xlo, xhi, ylo, yhi, ex, ey;
+            bool swapped;
+            const bool set = setup<KIND>(v0, v1, v2, p.W, p.H, area, xlo, xhi, ylo, yhi, ex, ey, false, &swapped);
+            const bool drawn = t < ntri && ((o0 | o1 | o2) & kBadVertex) == 0 && (o0 & o1 & o2) == 0;
+            const bool clip = drawn && (o0 | o1 | o2) != 0;
+            const bool inside = drawn && (o0 | o1 | o2) == 0 && set;
+            // triangles over the medium extent: spread over many blocks by the large-triangle raster
+            const bool big = inside && max(ex, ey) > ps.medium_extent;
+            const bool front = inside && !big;
+            if (front) {
+                if (swapped) packed = i0 | (i2 << 8) | (i1 << 16);
+                xlo_m = min(xlo_m, xlo);
+                ylo_m = min(ylo_m, ylo);
+                xhi_m = max(xhi_m, xhi);
+                yhi_m = max(yhi_m, yhi);
+                work += uint32_t((xhi - xlo + 1) * (yhi - ylo + 1));
             }
             const uint32_t cslot = wave_append(clip_counter, clip);
             if (clip) {
@@ -187,7 +180,7 @@ __global__ void __launch_bounds__(64) LH_RASTER_OCCUPANCY k_bin_meshlets(lh_scen
             const uint32_t bslot = wave_append(big_counter, big);
             if (big) {
                 if (bslot < v.big_capacity)
-                    store_big(qs.big + bslot, bv0, bv1, bv2, (mid << LH_KEY_TRI_BITS) | t, bx0, bx1, by0, by1, pi, false);
+                    store_big(qs.big + bslot, v0, v1, v2, (mid << LH_KEY_TRI_BITS) | t, xlo, xhi, ylo, yhi, pi, false);
                 else
                     atomicOr(&v.counters[LH_CNT_OVERFLOW], LH_OVERFLOW_BIG);
             }
