@@ -10,7 +10,11 @@
 //   lhp_copy                   a plain 16-byte-per-lane copy, the reference
 //                              kernel of known traffic that scales the HBM
 //                              counters (FETCH_SIZE, WRITE_SIZE).
+//   lhp_rcp_check              counts the floats of a range where rcp_exact
+//                              differs from the IEEE quotient 1.0f / a.
 #include <hip/hip_runtime.h>
+
+#include "lh_device.h"
 
 #include <cstdint>
 
@@ -57,6 +61,16 @@ __global__ void __launch_bounds__(256) k_copy(const uint4* __restrict__ src, uin
         dst[i] = src[i];
 }
 
+// Floats with bit patterns [lo, lo + n): one per lane of a grid-stride loop.
+__global__ void __launch_bounds__(256) k_rcp_check(uint32_t lo, uint32_t n, unsigned long long* mismatches) {
+    uint32_t bad = 0;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const float a = __uint_as_float(lo + i);
+        bad += __float_as_uint(lhd::rcp_exact(a)) != __float_as_uint(1.0f / a) ? 1u : 0u;
+    }
+    if (bad) atomicAdd(mismatches, (unsigned long long)bad);
+}
+
 }  // namespace
 
 extern "C" {
@@ -85,6 +99,15 @@ int lhp_copy(const void* src, void* dst, uint64_t bytes, void* stream) {
     const size_t n = bytes / 16;
     hipLaunchKernelGGL(k_copy, dim3(4096), dim3(256), 0, static_cast<hipStream_t>(stream),
                        static_cast<const uint4*>(src), static_cast<uint4*>(dst), n);
+    return hipGetLastError() == hipSuccess ? 0 : 3;
+}
+
+// Adds to *mismatches (device memory) the number of floats with bit patterns
+// in [lo_bits, hi_bits] where rcp_exact(a) != 1.0f / a.
+int lhp_rcp_check(uint32_t lo_bits, uint32_t hi_bits, unsigned long long* mismatches, void* stream) {
+    if (!mismatches || hi_bits < lo_bits || hi_bits - lo_bits == 0xFFFFFFFFu) return 1;
+    hipLaunchKernelGGL(k_rcp_check, dim3(8192), dim3(256), 0, static_cast<hipStream_t>(stream), lo_bits,
+                       hi_bits - lo_bits + 1, mismatches);
     return hipGetLastError() == hipSuccess ? 0 : 3;
 }
 

@@ -221,7 +221,7 @@ LH_DEV void raster_small(const lh_scene& s, const lh_view& v, const Pass& p, SV 
                          int32_t xlo, int32_t xhi, int32_t ylo, int32_t yhi, uint32_t id, Tally<ST>& tally) {
     const bool tl0 = top_left(v1.X, v1.Y, v2.X, v2.Y), tl1 = top_left(v2.X, v2.Y, v0.X, v0.Y),
                tl2 = top_left(v0.X, v0.Y, v1.X, v1.Y);
-    const float inv_area = 1.0f / float(area);
+    const float inv_area = rcp_exact(float(area));
     const float d10 = v1.z - v0.z, d20 = v2.z - v0.z;
     // E_i(Px + 256) = E_i(Px) - 256 (Yb - Ya); E_i(Py + 256) = E_i(Py) + 256 (Xb - Xa).
     const int32_t sx0 = -(v2.Y - v1.Y) * 256, sx1 = -(v0.Y - v2.Y) * 256, sx2 = -(v1.Y - v0.Y) * 256;
@@ -253,7 +253,7 @@ LH_DEV void raster_wave(const lh_scene& s, const lh_view& v, const Pass& p, SV v
     const int32_t area = (v1.X - v0.X) * (v2.Y - v0.Y) - (v1.Y - v0.Y) * (v2.X - v0.X);
     const bool tl0 = top_left(v1.X, v1.Y, v2.X, v2.Y), tl1 = top_left(v2.X, v2.Y, v0.X, v0.Y),
                tl2 = top_left(v0.X, v0.Y, v1.X, v1.Y);
-    const float inv_area = 1.0f / float(area);
+    const float inv_area = rcp_exact(float(area));
     const float d10 = v1.z - v0.z, d20 = v2.z - v0.z;
     const int32_t lx = int32_t(lane & 7), ly = int32_t(lane >> 3);
     for (int32_t ty = ylo; ty <= yhi; ty += 8) {

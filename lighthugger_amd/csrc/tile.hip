@@ -604,7 +604,7 @@ __global__ void __launch_bounds__(kTileThreads) LH_WAVES_ATTR(8) k_tile_raster(O
                         w = xhi - xlo + 1;
                         const int32_t h = yhi - ylo + 1;
                         // positive area (oriented by the binning), every value within int32 (<= 64 px)
-                        inv_area = 1.0f / float((b.X - a.X) * (c.Y - a.Y) - (b.Y - a.Y) * (c.X - a.X));
+                        inv_area = rcp_exact(float((b.X - a.X) * (c.Y - a.Y) - (b.Y - a.Y) * (c.X - a.X)));
                         if (max(w, h) * 256 <= ps.small_extent)
                             tile_lane<ST>(tile, ox, oy, a, b, c, xlo, xhi, ylo, yhi, inv_area,
                                           (mid << LH_KEY_TRI_BITS) | (tri >> 24), tally);
