@@ -150,13 +150,14 @@ def tile_algorithmic_bytes(scene, view, sizes, covered_pixels):
     per (meshlet, tile) entry the meshlet's snapped vertices from the bin cache
     (12 per vertex), its triangles of the entry (4 each), the header words and
     the entry itself (12); one 8-byte key per covered pixel written."""
+    from lighthugger_amd import abi
+
     cfg = view.cfg
     L = abi.scratch_layout(cfg.width, cfg.height, view.dcap, cfg.big_capacity, cfg.clip_capacity, view.bcap)
-    t = view.handle[0]
     base = view.offsets["scratch"]
 
     def words(off, count):
-        return t[base + off: base + off + 4 * count].cpu().numpy().view(np.uint32)
+        return view.mem.read(view.handle, base + off, 4 * count).view(np.uint32)
 
     n_items = int(view.counters()[abi.CNT_TILE_ITEMS])
     if n_items == 0:
