@@ -23,7 +23,7 @@ TUNING_FILE = Path(__file__).resolve().parent / "tuning.json"
 
 SPACE = {
     "small_extent_px": [2, 3, 4, 6, 8],
-    "medium_extent_px": [16, 32, 64],
+    "medium_extent_px": [8, 12, 16, 32, 64],
     "meshlet_blocks": [4096, 8192, 16384],
     "big_blocks": [512, 2048],
     "clip_blocks": [64, 256],
