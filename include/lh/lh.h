@@ -339,6 +339,9 @@ typedef struct lh_tuning {
                                    is non-zero); 0: the per-meshlet raster.  The tiled raster wins when
                                    meshlets cover many pixels each; with a few pixels per meshlet (small
                                    views of dense scenes) the per-meshlet raster is cheaper. */
+    uint32_t shadow_small_extent_px;  /* the cascades' small_extent_px (0..32; 0 = small_extent_px): the
+                                         shadow maps' triangles and raster differ from the main view's */
+    uint32_t shadow_medium_extent_px; /* the cascades' medium_extent_px (0..64; 0 = medium_extent_px) */
 } lh_tuning;
 /* lh_set_tuning sets every class at once, lh_get_tuning reads class 1;
  * lh_set_tuning_class / lh_get_tuning_class set or read one class.  Classes by

@@ -31,10 +31,14 @@ SPACE = {
     "tile_item_entries": [32, 64, 128],
     "tile_blocks": [768, 1024, 2048],
     "tile_raster": [0, 1],
+    "shadow_small_extent_px": [0, 4, 6, 8],
+    "shadow_medium_extent_px": [0, 8, 16, 32],
 }
 FIELDS = ("small_extent_px", "meshlet_blocks", "big_blocks", "clip_blocks", "cull_blocks", "medium_extent_px",
-          "tile_item_entries", "tile_blocks", "tile_raster")
-DEFAULTS = {"tile_item_entries": 64, "tile_blocks": 1024, "tile_raster": 1}  # fields absent from older files
+          "tile_item_entries", "tile_blocks", "tile_raster", "shadow_small_extent_px", "shadow_medium_extent_px")
+# fields absent from older files (0: the cascades take the main view's extents)
+DEFAULTS = {"tile_item_entries": 64, "tile_blocks": 1024, "tile_raster": 1, "shadow_small_extent_px": 0,
+            "shadow_medium_extent_px": 0}
 
 
 def _tuning(d: dict):
@@ -107,6 +111,9 @@ def search(cls: int, run, frames: int, sweeps: int = 2, log=print) -> dict:
                     continue
                 cand = dict(cur, **{field: val})
                 if cand["medium_extent_px"] < cand["small_extent_px"]:
+                    continue
+                shadow_small = cand["shadow_small_extent_px"] or cand["small_extent_px"]
+                if cand["shadow_medium_extent_px"] and cand["shadow_medium_extent_px"] < shadow_small:
                     continue
                 _set_class(cls, cand)
                 ms = _timer(run, frames)

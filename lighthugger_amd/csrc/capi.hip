@@ -169,7 +169,10 @@ lh_status lh_set_tuning_class(uint32_t cls, const lh_tuning* t) {
     if (cls >= LH_TUNING_CLASSES || !t || t->small_extent_px == 0 || t->small_extent_px > 32 ||
         t->meshlet_blocks == 0 || t->big_blocks == 0 || t->clip_blocks == 0 || t->cull_blocks == 0 ||
         t->medium_extent_px < t->small_extent_px || t->medium_extent_px > 64 ||
-        (t->tile_item_entries && (t->tile_item_entries < 8 || t->tile_item_entries > 512)) || t->tile_raster > 1)
+        (t->tile_item_entries && (t->tile_item_entries < 8 || t->tile_item_entries > 512)) || t->tile_raster > 1 ||
+        t->shadow_small_extent_px > 32 || t->shadow_medium_extent_px > 64 ||
+        (t->shadow_medium_extent_px &&
+         t->shadow_medium_extent_px < (t->shadow_small_extent_px ? t->shadow_small_extent_px : t->small_extent_px)))
         return LH_ERR_INVALID;
     lhk::Tuning& g = lhk::g_tunings[cls];
     g.raster.small_extent_px = t->small_extent_px;
@@ -177,6 +180,8 @@ lh_status lh_set_tuning_class(uint32_t cls, const lh_tuning* t) {
     g.raster.tile_item_entries = t->tile_item_entries ? t->tile_item_entries : 64u;
     g.raster.tile_blocks = t->tile_blocks ? t->tile_blocks : 1024u;
     g.raster.tile_raster = t->tile_raster;
+    g.raster.shadow_small_extent_px = t->shadow_small_extent_px;
+    g.raster.shadow_medium_extent_px = t->shadow_medium_extent_px;
     g.raster.meshlet_blocks = t->meshlet_blocks;
     g.raster.big_blocks = t->big_blocks;
     g.raster.clip_blocks = t->clip_blocks;
@@ -192,6 +197,8 @@ lh_status lh_get_tuning_class(uint32_t cls, lh_tuning* t) {
     t->tile_item_entries = g.raster.tile_item_entries;
     t->tile_blocks = g.raster.tile_blocks;
     t->tile_raster = g.raster.tile_raster;
+    t->shadow_small_extent_px = g.raster.shadow_small_extent_px;
+    t->shadow_medium_extent_px = g.raster.shadow_medium_extent_px;
     t->meshlet_blocks = g.raster.meshlet_blocks;
     t->big_blocks = g.raster.big_blocks;
     t->clip_blocks = g.raster.clip_blocks;

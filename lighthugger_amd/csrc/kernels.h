@@ -22,6 +22,8 @@ struct RasterTuning {
     uint32_t tile_item_entries = 64; // tiled main-view raster: binned meshlets per work item (>= 8)
     uint32_t tile_blocks = 1024;    // tiled main-view raster: persistent grid (512-thread blocks)
     uint32_t tile_raster = 1;       // main view through the tiled raster (when bin_capacity > 0)
+    uint32_t shadow_small_extent_px = 0;  // the cascades' small_extent_px (0: the same)
+    uint32_t shadow_medium_extent_px = 0; // the cascades' medium_extent_px (0: the same)
 };
 
 struct Tuning {

@@ -430,8 +430,13 @@ hipError_t raster(const lh_scene& s, const VS& vs, uint32_t pass0, uint32_t npas
                   hipStream_t st, bool phase2) {
     if (npasses == 0 || npasses > LH_NUM_CASCADES || pass0 + npasses > LH_NUM_PASSES || (pass0 == 0 && npasses != 1))
         return hipErrorInvalidValue;
-    const uint32_t small_px = std::min<uint32_t>(std::max<uint32_t>(t.small_extent_px, 1), 32);
-    const uint32_t medium_px = std::min<uint32_t>(std::max(t.medium_extent_px, small_px), 64);
+    // the cascades may take their own extents (their triangles are smaller
+    // and their raster depth-only)
+    const bool cascades = pass0 > 0;
+    const uint32_t small_sel = cascades && t.shadow_small_extent_px ? t.shadow_small_extent_px : t.small_extent_px;
+    const uint32_t medium_sel = cascades && t.shadow_medium_extent_px ? t.shadow_medium_extent_px : t.medium_extent_px;
+    const uint32_t small_px = std::min<uint32_t>(std::max<uint32_t>(small_sel, 1), 32);
+    const uint32_t medium_px = std::min<uint32_t>(std::max(medium_sel, small_px), 64);
     PassSet ps{pass0, npasses, int32_t(small_px * 256), int32_t(medium_px * 256), phase2 ? 1u : 0u, 0u};
     const uint32_t nv = vs.count();
     // per-view persistent grids of the queue kernels shrink with the batch size

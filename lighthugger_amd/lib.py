@@ -31,7 +31,8 @@ class Tuning(C.Structure):
 
     _fields_ = [("small_extent_px", C.c_uint32), ("meshlet_blocks", C.c_uint32), ("big_blocks", C.c_uint32),
                 ("clip_blocks", C.c_uint32), ("cull_blocks", C.c_uint32), ("medium_extent_px", C.c_uint32),
-                ("tile_item_entries", C.c_uint32), ("tile_blocks", C.c_uint32), ("tile_raster", C.c_uint32)]
+                ("tile_item_entries", C.c_uint32), ("tile_blocks", C.c_uint32), ("tile_raster", C.c_uint32),
+                ("shadow_small_extent_px", C.c_uint32), ("shadow_medium_extent_px", C.c_uint32)]
 
     def as_dict(self) -> dict:
         return {f: int(getattr(self, f)) for f, _ in self._fields_}

@@ -117,6 +117,12 @@ def test_host_only_entry_points_without_gpu():
     assert L.lh_set_tuning(C.byref(bad)) == abi.LH_ERR_INVALID
     bad = Tuning(8, 1, 1, 1, 1, 8, 0, 0, 2)  # tile_raster is 0 or 1
     assert L.lh_set_tuning(C.byref(bad)) == abi.LH_ERR_INVALID
+    bad = Tuning(8, 1, 1, 1, 1, 8, 0, 0, 1, 33)  # the cascades' small extent is at most 32
+    assert L.lh_set_tuning(C.byref(bad)) == abi.LH_ERR_INVALID
+    bad = Tuning(8, 1, 1, 1, 1, 8, 0, 0, 1, 6, 4)  # the cascades' medium extent below their small one
+    assert L.lh_set_tuning(C.byref(bad)) == abi.LH_ERR_INVALID
+    bad = Tuning(8, 1, 1, 1, 1, 16, 0, 0, 1, 0, 4)  # ... or below the main view's small one it inherits
+    assert L.lh_set_tuning(C.byref(bad)) == abi.LH_ERR_INVALID
     classes = get_tuning_classes()
     set_tuning_classes(classes)
     assert [c.as_dict() for c in get_tuning_classes()] == [c.as_dict() for c in classes]

@@ -276,8 +276,12 @@ def test_output_independent_of_tuning():
     saved = get_tuning_classes()
     ref = None
     try:
+        # the last two give the cascades extents of their own (every shadow
+        # triangle in-lane up to 32 px; none in-lane and none wave-cooperative)
         for t in (Tuning(1, 8192, 2048, 256, 1024, 1, 8, 256, 1), Tuning(2, 4096, 512, 64, 512, 16, 512, 2048, 1),
-                  Tuning(8, 16384, 2048, 256, 2048, 64, 32, 64, 0), Tuning(32, 1024, 64, 16, 128, 64, 0, 0, 1)):
+                  Tuning(8, 16384, 2048, 256, 2048, 64, 32, 64, 0), Tuning(32, 1024, 64, 16, 128, 64, 0, 0, 1),
+                  Tuning(4, 8192, 2048, 256, 1024, 8, 64, 1024, 1, 32, 64),
+                  Tuning(16, 8192, 2048, 256, 1024, 64, 64, 1024, 1, 1, 1)):
             set_tuning(t)
             r.frame()
             torch.cuda.synchronize()
