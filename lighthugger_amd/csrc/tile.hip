@@ -26,8 +26,9 @@
 //   k_bin_fill      one thread per draw: writes its entries.
 //   k_tile_raster   persistent blocks of 8 waves over the items: LDS tile
 //                   cleared, each wave rasterizes one binned meshlet at a time
-//                   from the cache (ds_min_u64 per covered sample), then the
-//                   tile's covered pixels are written out.
+//                   (two when their triangles fit one round) from the cache
+//                   (ds_min_u64 per covered sample), then the tile's covered
+//                   pixels are written out.
 // The per-meshlet raster then runs over the direct list, followed by the
 // alpha-clip, clip and large-triangle launches (all device-scope atomicMin on
 // top of what the tiles wrote).
@@ -475,6 +476,7 @@ LH_DEV void tile_expand(uint64_t* tile, int32_t ox, int32_t oy, int32_t* own, co
         const int32_t obox = __shfl(box, o, 64);
         const uint32_t oex = __shfl(excl, o, 64);
         const float oinv = __shfl(inv_area, o, 64);
+        const uint32_t omid = uint32_t(__shfl(int(mid), o, 64));
         const uint32_t k = base + lane;
         if (k < total) {
             // run (local % wr, local / wr) of the owner's box; local < 4096, wr <= 16
@@ -497,7 +499,7 @@ LH_DEV void tile_expand(uint64_t* tile, int32_t ox, int32_t oy, int32_t* own, co
                  tl2 = top_left(a.X, a.Y, b.X, b.Y);
             float z0 = a.z, d10 = b.z - a.z, d20 = c.z - a.z, inv = oinv;
             pin(z0), pin(d10), pin(d20), pin(inv), pin(s0), pin(s1), pin(s2);
-            const uint32_t id = (mid << LH_KEY_TRI_BITS) | (otri >> 24);
+            const uint32_t id = (omid << LH_KEY_TRI_BITS) | (otri >> 24);
             for (int32_t j = 0; j < len; j++) {
                 if (inside(e0, e1, e2, tl0, tl1, tl2)) {
                     const float l1 = float(e1) * inv, l2 = float(e2) * inv;
@@ -514,10 +516,15 @@ LH_DEV void tile_expand(uint64_t* tile, int32_t ox, int32_t oy, int32_t* own, co
 // Persistent blocks of kTileWaves waves over the work items.  Item: clear the
 // LDS tile; each wave takes every kTileWaves-th entry (one binned meshlet),
 // with the entry list of the item held one per lane and the next meshlet's
-// cache lines requested while the current one is rasterized; then the
-// covered pixels go out: a plain store when the item is the tile's only one
-// (the visibility buffer holds the clear value there, as nothing else has
-// drawn yet), an atomicMin otherwise and in the second occlusion phase.
+// cache lines requested while the current one is rasterized.  When an entry
+// and the next hold at most 64 triangles together (an entry averages about
+// 35), both go through one round: the second's vertices at LDS slots
+// 64..127, its triangles on the lanes after the first's, each lane with its
+// own meshlet id.  Then the covered pixels go out: a plain store when the
+// item is the tile's only one (the visibility buffer holds the clear value
+// there, as nothing else has drawn yet), an atomicMin otherwise and in the
+// second occlusion phase.  Six waves per SIMD (80 VGPRs; the pairing's
+// registers would otherwise take the kernel to five).
 struct CacheLanes {
     int32_t X, Y;
     float Z;
@@ -538,10 +545,10 @@ LH_DEV CacheLanes load_cache(const BinCache& c, uint32_t lane, uint32_t first) {
 }
 
 template <bool ST>
-__global__ void __launch_bounds__(kTileThreads) LH_WAVES_ATTR(8) k_tile_raster(OneView vs, PassSet ps) {
+__global__ void __launch_bounds__(kTileThreads) LH_WAVES_ATTR(6) k_tile_raster(OneView vs, PassSet ps) {
     __shared__ uint64_t tile[kTile * kTile];
-    __shared__ int32_t sX[kTileWaves][64], sY[kTileWaves][64];
-    __shared__ float sZ[kTileWaves][64];
+    __shared__ int32_t sX[kTileWaves][128], sY[kTileWaves][128];
+    __shared__ float sZ[kTileWaves][128];
     __shared__ int32_t own[kTileWaves][64];
     const lh_view& v = vs.v;
     const Bins B = bins(v);
@@ -576,8 +583,11 @@ __global__ void __launch_bounds__(kTileThreads) LH_WAVES_ATTR(8) k_tile_raster(O
             cur = load_cache(B.cache[uint32_t(readlane(int32_t(my_g), 0))], lane,
                              uint32_t(readlane(int32_t(my_first), 0)));
         __syncthreads();
-        for (uint32_t k = 0; k < mine; k++) {
+        for (uint32_t k = 0; k < mine;) {
             const uint32_t ntri = uint32_t(readlane(int32_t(my_count), k)), mid = uint32_t(readlane(int32_t(my_mid), k));
+            const uint32_t ntri2 = k + 1 < mine ? uint32_t(readlane(int32_t(my_count), k + 1)) : 65u;
+            const bool pair = ntri + ntri2 <= 64u;  // wave-uniform
+            const uint32_t mid2 = pair ? uint32_t(readlane(int32_t(my_mid), k + 1)) : 0u;
             CacheLanes next{};
             if (k + 1 < mine)
                 next = load_cache(B.cache[uint32_t(readlane(int32_t(my_g), k + 1))], lane,
@@ -585,15 +595,27 @@ __global__ void __launch_bounds__(kTileThreads) LH_WAVES_ATTR(8) k_tile_raster(O
             sX[wave][lane] = cur.X;
             sY[wave][lane] = cur.Y;
             sZ[wave][lane] = cur.Z;
+            uint32_t tri_pair = cur.t0, lane_mid = mid;
+            if (pair) {
+                sX[wave][64 + lane] = next.X;
+                sY[wave][64 + lane] = next.Y;
+                sZ[wave][64 + lane] = next.Z;
+                const uint32_t from = uint32_t(__shfl(int(next.t0), int((lane - ntri) & 63u), 64));
+                if (lane >= ntri) {
+                    tri_pair = from + 0x404040u;  // corners of the second entry at slots 64..127
+                    lane_mid = mid2;
+                }
+            }
+            const uint32_t total = pair ? ntri + ntri2 : ntri;
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
             __builtin_amdgcn_wave_barrier();
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-            for (uint32_t round = 0; round < 2 && round * 64 < ntri; round++) {
-                const uint32_t tri = round ? cur.t1 : cur.t0;
+            for (uint32_t round = 0; round < 2 && round * 64 < total; round++) {
+                const uint32_t tri = pair ? tri_pair : (round ? cur.t1 : cur.t0);
                 uint32_t n = 0;
                 int32_t xlo = 0, ylo = 0, w = 1;
                 float inv_area = 0.0f;
-                if (lane + 64 * round < ntri) {
+                if (lane + 64 * round < total) {
                     const uint32_t i0 = tri & 255u, i1 = (tri >> 8) & 255u, i2 = (tri >> 16) & 255u;
                     const SV a{sX[wave][i0], sY[wave][i0], sZ[wave][i0]};
                     const SV b{sX[wave][i1], sY[wave][i1], sZ[wave][i1]};
@@ -611,15 +633,23 @@ __global__ void __launch_bounds__(kTileThreads) LH_WAVES_ATTR(8) k_tile_raster(O
                         inv_area = rcp_exact(float((b.X - a.X) * (c.Y - a.Y) - (b.Y - a.Y) * (c.X - a.X)));
                         if (max(w, h) * 256 <= ps.small_extent)
                             tile_lane<ST>(tile, ox, oy, a, b, c, xlo, xhi, ylo, yhi, inv_area,
-                                          (mid << LH_KEY_TRI_BITS) | (tri >> 24), tally);
+                                          (lane_mid << LH_KEY_TRI_BITS) | (tri >> 24), tally);
                         else
                             n = uint32_t(runs_per_row(w) * h);
                     }
                 }
                 tile_expand<ST>(tile, ox, oy, own[wave], sX[wave], sY[wave], sZ[wave], tri, n, xlo, ylo, w, inv_area,
-                                mid, lane, tally);
+                                lane_mid, lane, tally);
             }
-            cur = next;
+            if (pair) {
+                k += 2;  // the entry after the pair was not requested yet
+                if (k < mine)
+                    cur = load_cache(B.cache[uint32_t(readlane(int32_t(my_g), k))], lane,
+                                     uint32_t(readlane(int32_t(my_first), k)));
+            } else {
+                k += 1;
+                cur = next;
+            }
         }
         tally.flush(v, 0);
         __syncthreads();
