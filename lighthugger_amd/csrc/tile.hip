@@ -605,6 +605,11 @@ __global__ void __launch_bounds__(kTileThreads) LH_WAVES_ATTR(6) k_tile_raster(O
                     tri_pair = from + 0x404040u;  // corners of the second entry at slots 64..127
                     lane_mid = mid2;
                 }
+                // the second entry is consumed: its registers take the request
+                // of the entry after the pair
+                if (k + 2 < mine)
+                    next = load_cache(B.cache[uint32_t(readlane(int32_t(my_g), k + 2))], lane,
+                                      uint32_t(readlane(int32_t(my_first), k + 2)));
             }
             const uint32_t total = pair ? ntri + ntri2 : ntri;
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -641,15 +646,8 @@ __global__ void __launch_bounds__(kTileThreads) LH_WAVES_ATTR(6) k_tile_raster(O
                 tile_expand<ST>(tile, ox, oy, own[wave], sX[wave], sY[wave], sZ[wave], tri, n, xlo, ylo, w, inv_area,
                                 lane_mid, lane, tally);
             }
-            if (pair) {
-                k += 2;  // the entry after the pair was not requested yet
-                if (k < mine)
-                    cur = load_cache(B.cache[uint32_t(readlane(int32_t(my_g), k))], lane,
-                                     uint32_t(readlane(int32_t(my_first), k)));
-            } else {
-                k += 1;
-                cur = next;
-            }
+            k += pair ? 2u : 1u;
+            cur = next;
         }
         tally.flush(v, 0);
         __syncthreads();
