@@ -243,11 +243,6 @@ __global__ void __launch_bounds__(64) k_raster_clip(lh_scene s, VS vs, PassSet p
     }
 }
 
-// View blockIdx.z; one 256-thread block per queued triangle; the bbox is
-// walked in 16x16 tiles and each pixel evaluates the int64 edge functions.
-// ALPHA selects the entries of alpha-clip geometry (alpha test per sample),
-// so the opaque instantiation keeps its small register footprint; the alpha
-// one returns at once when the view's culls counted no alpha-clip draw.
 // Triangles of the large queue whose pixel box holds at most this many pixels
 // are rasterized by one wave alone; larger ones by the whole block.
 constexpr int32_t kWaveBoxPx = 4096;
@@ -295,11 +290,14 @@ LH_DEV void raster_box(const lh_scene& s, const lh_view& v, const PassSet& ps, c
     }
 }
 
-// Each round the block takes four queue entries, one per wave.  A wave
-// rasterizes its own triangle when the box is at most kWaveBoxPx pixels (most
-// of the queue: slivers just over the medium extent), so a 54-pixel box costs
-// one wave pass instead of a 256-thread block; the larger boxes of the round
-// are rasterized by all four waves together first.
+// View blockIdx.z.  Each round the block takes four queue entries, one per
+// wave.  A wave rasterizes its own triangle when the box is at most
+// kWaveBoxPx pixels (most of the queue: slivers just over the medium extent),
+// so a 54-pixel box costs one wave pass instead of a 256-thread block; the
+// larger boxes of the round are rasterized by all four waves together first.
+// ALPHA selects the entries of alpha-clip geometry (alpha test per sample),
+// so the opaque instantiation keeps its small register footprint; the alpha
+// one returns at once when the view's culls counted no alpha-clip draw.
 template <int KIND, class VS, bool ST, bool ALPHA>
 __global__ void __launch_bounds__(256) k_raster_big(lh_scene s, VS vs, PassSet ps) {
     const lh_view& v = vs.get(blockIdx.z);
