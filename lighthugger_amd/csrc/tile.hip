@@ -290,7 +290,6 @@ __global__ void __launch_bounds__(kScanThreads) k_bin_scan(OneView vs, uint32_t 
     const lh_view& v = vs.v;
     const Bins B = bins(v);
     const uint32_t tiles = B.L.tiles, lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
-    constexpr uint32_t kWaves = kScanThreads / 64;
     if (threadIdx.x == 0) carry[0] = carry[1] = 0;
     __syncthreads();
     for (uint32_t t0 = 0; t0 < tiles; t0 += kScanThreads) {
