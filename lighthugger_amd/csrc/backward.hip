@@ -49,7 +49,6 @@ LH_DEV d3 operator-(d3 a, d3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
 LH_DEV d3 operator*(d3 a, double s) { return {a.x * s, a.y * s, a.z * s}; }
 LH_DEV double dot(d3 a, d3 b) { return (a.x * b.x + a.y * b.y) + a.z * b.z; }
 LH_DEV d3 cross(d3 a, d3 b) { return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x}; }
-LH_DEV double len(d3 a) { return sqrt(dot(a, a)); }
 // columns 0..2 of a 4x4 (column-major) applied to a point / the linear part
 LH_DEV d3 col3(const LhVec4& c) { return {double(c.x), double(c.y), double(c.z)}; }
 
