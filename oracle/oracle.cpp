@@ -851,7 +851,6 @@ F3 temperature(float t) {
 // render_geometry main() (src/shaders/render_geometry.comp:70-266) for one pixel.
 F3 resolve_pixel(const lh_scene& sc, const lh_view& v, uint32_t x, uint32_t y) {
     const Uniforms& u = *v.uniforms;
-    const MiscStorage& misc = *v.misc;
     F2 win{float(u.window_size.x), float(u.window_size.y)};
     F2 ndc{float(x) / win.x * 2.0f - 1.0f, float(y) / win.y * 2.0f - 1.0f};  // pixel corner (:82-84)
     uint64_t key = v.visbuffer[size_t(y) * v.width + x];
